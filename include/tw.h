@@ -1,0 +1,220 @@
+/*
+ * tw.h -- C-ABI of the MI355X-native staggered SBP-SAT wave-solver hot path.
+ *
+ * The reference (wsmoses/tpu-wave) exposes no library: its callable surface is
+ * the C++ member functions that its three main() drivers call.  Each entry point
+ * below replaces one of those, with the reference function it stands in for
+ * cited beside it (paths relative to the reference tree).
+ *
+ * Conventions
+ *   - Data axis x is the slow (row) axis, y the fast (column) axis; physical
+ *     'Y' (depth) maps to data x and physical 'X' to data y, as in
+ *     cuda/src/cpp/namespace_forward.hpp:56-61 (XY_to_01).
+ *   - Staggered grid types (type along x, type along y): 'N' = nodes (M+1
+ *     points), 'M' = cell centres (M points).  Grid ids follow the reference's
+ *     Array_Grid_types order NN, MN, NM, MM (namespace_forward.hpp:159-167).
+ *   - Fields are named by data axes:  TW_VX lives on NM (the reference's "Vy"
+ *     grid), TW_VY on MN (the reference's "Vx" grid), TW_SXX/TW_SYY on MM
+ *     (SMM soln[0]/soln[1]), TW_SXY on NN, TW_P on MM (acoustic).
+ *   - Every function returns 0 on success and a nonzero TW_E* code otherwise;
+ *     nothing calls exit().  tw_last_error() gives the message.
+ *   - A solver handle owns all of its device memory; caller arrays are
+ *     borrowed for the duration of the call only.  A handle is not
+ *     thread-safe; distinct handles are independent.
+ *   - Host arrays are dense row-major [rows][cols] doubles of the field's
+ *     grid size (see tw_field_shape).
+ */
+#ifndef TW_H
+#define TW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tw_solver tw_solver;
+
+/* ---- error codes ---- */
+#define TW_OK          0
+#define TW_EINVAL      1   /* bad argument / inconsistent configuration          */
+#define TW_ERANGE      2   /* index or location outside the grid / weak boundary  */
+#define TW_ENOMEM      3   /* allocation failed or over the size guard           */
+#define TW_EDEVICE     4   /* HIP runtime error                                  */
+#define TW_ENONFINITE  5   /* non-finite value detected in the state             */
+#define TW_ESTATE      6   /* call not valid in the current state                */
+
+/* ---- enums ---- */
+enum tw_physics   { TW_ELASTIC = 0, TW_ACOUSTIC = 1 };
+enum tw_precision { TW_FP64 = 0, TW_FP32 = 1, TW_BF16 = 2, TW_FP16 = 3 };
+/* compensation variants of the leapfrog update S <- S + R
+ * (cuda/src/cuda/grid_update.cut:62-124, cuda/src/cpp/grid_update.tpp:1-128):
+ *   TW_CPST_0   S += R, R = 0                      ('0')
+ *   TW_CPST_3R  fast2sum(S,R) -> (S,R)             ( 3,'R')
+ *   TW_CPST_M3R explicit 3-flop form of the above  (-3,'R')
+ *   TW_CPST_6R  2Sum(S,R) -> (S,R)                 ( 6,'R')
+ *   TW_CPST_3C  fast2sum(S,R+C) -> (S,C), R = 0    ( 3,'C')
+ *   TW_CPST_M3C explicit 3-flop form of the above  (-3,'C')
+ *   TW_CPST_6C  2Sum(S,R+C) -> (S,C), R = 0        ( 6,'C')
+ *   TW_CPST_K   16-bit storage only: FP32 sum, the rounding residual of the
+ *               store is kept in a 16-bit companion array (Kahan-style) */
+enum tw_cpst {
+  TW_CPST_0 = 0, TW_CPST_3R = 1, TW_CPST_M3R = 2, TW_CPST_6R = 3,
+  TW_CPST_3C = 4, TW_CPST_M3C = 5, TW_CPST_6C = 6, TW_CPST_K = 7
+};
+enum tw_field { TW_VX = 0, TW_VY = 1, TW_SXX = 2, TW_SYY = 3, TW_SXY = 4, TW_P = 5 };
+enum tw_grid  { TW_GRID_NN = 0, TW_GRID_MN = 1, TW_GRID_NM = 2, TW_GRID_MM = 3 };
+/* material coefficients as used by the update (cuda/src/cpp/grid_parameter.cpp:75-162) */
+enum tw_coef  {
+  TW_C_BINV_VX = 0,  /* 1/rho on NM            */
+  TW_C_BINV_VY = 1,  /* 1/rho on MN            */
+  TW_C_MU      = 2,  /* mu on NN (elastic)     */
+  TW_C_LAMBDA  = 3,  /* lambda on MM (elastic) */
+  TW_C_TWOMU   = 4,  /* 2 mu on MM (elastic)   */
+  TW_C_KAPPA   = 5   /* kappa = rho vp^2 on MM (acoustic) */
+};
+
+typedef struct tw_config {
+  int32_t physics;        /* TW_ELASTIC or TW_ACOUSTIC                               */
+  int32_t mx, my;         /* cells along data x (rows) and data y (columns)          */
+  double  dx;             /* grid spacing (both axes)                                */
+  int64_t num_dx, den_dx; /* dx as num/den for rational locations; den_dx <= 0: none */
+  double  dt;             /* time step                                               */
+  char    bc[4];          /* boundary of sides xL, xR, yL, yR: 'P' periodic (both
+                             sides of an axis), 'F' free-surface SAT,
+                             'C' characteristic (absorbing) SAT                      */
+  int32_t precision;      /* field storage type (tw_precision); arithmetic is FP64
+                             for TW_FP64 and FP32 otherwise                          */
+  int32_t coef16;         /* BF16/FP16 only: store coefficients in the 16-bit type  */
+  int32_t cpst;           /* tw_cpst                                                 */
+  int32_t stencil_form;   /* 0: stencil*(dt/dx) (cuda/src/cpp/grid.hpp:281-282);
+                             1: stencil*dt/dx (singlecore/src/cpp/namespaces.hpp:607-608) */
+  int32_t energy_form;    /* 0: H-norm energy with compliance weights, /2
+                             (cuda/src/cpp/grid_energy.cpp:8-216);
+                             1: acoustic reference sum soln*prmt*soln*w with
+                             w = dx^2/2 (halved at node ends), no /2
+                             (singlecore/src/cpp/namespaces.hpp:734-777,
+                              singlecore/src/main_cpp/cpu_acoustic.cpp:327-344)      */
+  int32_t record_energy;  /* 1: record energy every step                             */
+  int32_t nt_max;         /* capacity (steps) of the receiver and energy records     */
+  double  source_scaling; /* source increment is divided by it
+                             (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:554)          */
+  int32_t device;         /* HIP device ordinal                                      */
+} tw_config;
+
+/* ---- host-only helpers (no device needed) ---- */
+
+/* Fills cfg with defaults: elastic, free-surface x / periodic y, FP32, no
+ * compensation, energy on.  Sizes, dx, dt and nt_max must then be set. */
+void tw_config_default(tw_config* cfg);
+
+/* Checks a configuration; 0 if usable. Message via tw_global_error(). */
+int tw_config_check(const tw_config* cfg);
+
+/* Shape of a field's grid: rows = mx or mx+1, cols = my or my+1 by type. */
+int tw_field_shape(const tw_config* cfg, int32_t field, int32_t* rows, int32_t* cols);
+
+/* Rational physical location -> grid index on a grid type ("MN" etc. as
+ * data-x type, data-y type).  Replaces the index maps of
+ * Class_Forward_Specs::process_src_locations / process_rcv_locations
+ * (cuda/src/cpp/forward.cpp:30-54, 127-151): node index num*den_dx/(den*num_dx),
+ * cell index (2*num*den_dx - den*num_dx)/(2*den*num_dx); the division must be
+ * exact, otherwise TW_ERANGE. */
+int tw_locate(const tw_config* cfg, int32_t grid, int64_t num_x, int64_t den_x,
+              int64_t num_y, int64_t den_y, int32_t* ix, int32_t* iy);
+
+/* Bilinear interpolation of a node-based model (nxm+1) x (nym+1) with spacing
+ * num_dxm/den_dxm onto the points of one staggered grid.  Replaces
+ * Class_Grid::interpolate_forward_parameter (cuda/src/cpp/grid_parameter.cpp:6-60). */
+int tw_interpolate_model(const tw_config* cfg, int32_t grid, const double* model,
+                         int32_t nxm, int32_t nym, int64_t num_dxm, int64_t den_dxm,
+                         double* out);
+
+/* Ricker wavelet increment A*dt/dx^2/source_scaling for a step; replaces the
+ * source evaluation in forward_simulation_periodic_y
+ * (cuda/src/cpp/forward_simulation.cpp:62-70,104-112). half = 0 for velocity
+ * grids (t = it*dt - t0), 1 for stress/pressure grids (t = (it+1/2)*dt - t0). */
+double tw_ricker_increment(const tw_config* cfg, double f0, double t0, int32_t it, int32_t half);
+
+const char* tw_global_error(void);
+const char* tw_version(void);
+
+/* ---- solver lifecycle ---- */
+
+/* Grid setup: replaces Class_Grid::set_grid_parameters / set_forward_operators /
+ * set_grid_pointers (cuda/src/cpp/grid.hpp:146-481) and
+ * cuda_Class_Grid::cuda_Class_Grid_initialize (cuda/src/cuda/grid.cuh:216-408).
+ * Fields start at zero; coefficients default to rho = kappa = 1, vp = 2, vs = 1
+ * (the reference's verification material, nvcc_device_FDM_2D.cu:153-159). */
+int tw_create(const tw_config* cfg, tw_solver** out);
+int tw_destroy(tw_solver* s);
+const char* tw_last_error(const tw_solver* s);
+
+/* Material from a node model: replaces retrieve_forward_parameter,
+ * define_parameters_energy, adjust_parameters_energy_periodic and
+ * make_density_reciprocal (cuda/src/cpp/grid_parameter.cpp:75-162,
+ * cuda/src/cpp/grid_energy.cpp:8-177).  Arrays are (nxm+1) x (nym+1) nodes;
+ * for TW_ACOUSTIC vs is ignored and kappa = rho*vp^2. */
+int tw_set_model(tw_solver* s, const double* rho, const double* vp, const double* vs,
+                 int32_t nxm, int32_t nym, int64_t num_dxm, int64_t den_dxm);
+
+/* One coefficient array on its grid (tw_coef), overriding tw_set_model's. */
+int tw_set_coef(tw_solver* s, int32_t coef, const double* values);
+int tw_get_coef(tw_solver* s, int32_t coef, double* values);
+
+/* Point source on every field of a grid (cuda/src/cpp/forward.cpp:10-100 for
+ * the placement rules; forward_simulation.cpp:57-114 for the injection).
+ * TW_ERANGE if off-grid, or on an x node boundary of an SBP axis. */
+int tw_set_source_ricker(tw_solver* s, int32_t grid, int32_t ix, int32_t iy,
+                         double f0, double t0);
+/* Custom source: increment per step (already scaled), n >= steps run. */
+int tw_set_source_wavelet(tw_solver* s, int32_t grid, int32_t ix, int32_t iy,
+                          const double* increments, int32_t n);
+int tw_clear_source(tw_solver* s);
+
+/* Receivers, one field each (Map_grid_struct_rcv_forward,
+ * cuda/src/cpp/forward.cpp:104-211). Replaces any previous set. */
+int tw_set_receivers(tw_solver* s, int32_t n, const int32_t* field,
+                     const int32_t* ix, const int32_t* iy);
+
+/* Advance nsteps leapfrog steps: one iteration of the time loop of
+ * forward_simulation_periodic_y (cuda/src/cpp/forward_simulation.cpp:30-179) /
+ * nvcc_device_FDM_2D.cu:449-730 per step.  Asynchronous w.r.t. the host. */
+int tw_step(tw_solver* s, int32_t nsteps);
+int tw_synchronize(tw_solver* s);
+int tw_current_step(tw_solver* s, int32_t* it);
+
+/* Receiver traces [n][nt] (Map_grid_RESULT_rcv, nvcc_device_FDM_2D.cu:737-751)
+ * and energy records (Record_E_k/E_p0/E_p1, forward.cpp:288-327), nt = steps
+ * run since the last reset; multiplied back by source_scaling as the
+ * reference's writers do. Null pointers are skipped. */
+int tw_read_receivers(tw_solver* s, double* out, int32_t* nt);
+int tw_read_energy(tw_solver* s, double* ek, double* ep0, double* ep1, int32_t* nt);
+
+/* Field snapshot / initial condition (cuda_Class_Grid::copy_field_pitched,
+ * cuda/src/cuda/grid.cuh:414-512).  Values are in stored units (not rescaled). */
+int tw_get_field(tw_solver* s, int32_t field, double* out);
+int tw_set_field(tw_solver* s, int32_t field, const double* in);
+/* Companion array (R or C compensation, or the Kahan residual) of a field. */
+int tw_get_companion(tw_solver* s, int32_t field, double* out);
+
+/* Zero all fields, companions, records and the step counter. */
+int tw_reset(tw_solver* s);
+
+/* Checkpoint of the complete dynamic state (fields, companions, step index);
+ * restoring it and stepping reproduces an uninterrupted run bit for bit. */
+int tw_checkpoint_size(tw_solver* s, size_t* bytes);
+int tw_checkpoint_save(tw_solver* s, void* buf, size_t bytes);
+int tw_checkpoint_load(tw_solver* s, const void* buf, size_t bytes);
+
+/* Run on the caller's HIP stream (hipStream_t) instead of the handle's own. */
+int tw_set_stream(tw_solver* s, void* hip_stream);
+
+/* Check the state for non-finite values (TW_ENONFINITE if any). */
+int tw_check_finite(tw_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TW_H */

@@ -1,0 +1,106 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point declared in include/tw.h, and its host-only helpers validate input."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import twave as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "tw.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char\*)\s+(tw_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_header():
+    lib = T.lib()
+    decl = declared_symbols()
+    assert len(decl) >= 30
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert sorted(T.exported_symbols()) == decl
+
+
+def test_config_struct_layout_matches_oracle():
+    import oracle as O
+    assert ctypes.sizeof(T.Config) == ctypes.sizeof(O.TwConfig)
+    for (a, _), (b, _) in zip(T.Config._fields_, O.TwConfig._fields_):
+        assert a == b
+        assert getattr(T.Config, a).offset == getattr(O.TwConfig, b).offset
+
+
+def test_config_default_and_check():
+    c = T.Config()
+    T.lib().tw_config_default(ctypes.byref(c))
+    assert c.physics == T.ELASTIC and c.bc == b"FFPP" and c.precision == T.FP32
+    T.check_config(c)
+    for field, bad in (("mx", 4), ("dt", 0.0), ("precision", 9), ("cpst", 99), ("source_scaling", 0.0)):
+        d = T.config()
+        setattr(d, field, bad)
+        with pytest.raises(T.TwError):
+            T.check_config(d)
+    with pytest.raises(T.TwError):
+        T.check_config(T.config(bc="PFPP"))        # periodic must pair
+    with pytest.raises(T.TwError):
+        T.check_config(T.config(precision="fp32", cpst="K"))
+    with pytest.raises(T.TwError):
+        T.check_config(T.config(precision="bf16", cpst="6R"))
+    with pytest.raises(T.TwError):
+        T.check_config(T.config(energy_form=1))    # acoustic periodic only
+
+
+def test_field_shapes():
+    c = T.config(mx=10, my=12)
+    assert T.field_shape(c, T.VX) == (11, 12)
+    assert T.field_shape(c, T.VY) == (10, 13)
+    assert T.field_shape(c, T.SXX) == (10, 12)
+    assert T.field_shape(c, T.SXY) == (11, 13)
+    with pytest.raises(T.TwError):
+        T.field_shape(c, T.P)
+
+
+def test_locate_matches_oracle_and_kat():
+    """Shipped case (cuda/data/homogeneous/SrcRcv.txt): Src_1_MN_401_250_20_250
+    on 600x600, dx = 1/125 lands on data (N,M) at (10, 200), receiver
+    Rcv_1_1_MN_801_250_20_250 at (10, 400)."""
+    import oracle as O
+    c = T.config(mx=600, my=600, num_dx=1, den_dx=125)
+    # physical X -> data y, physical Y -> data x; "MN" physical = data (N, M) = GRID_NM
+    assert T.locate(c, T.GRID_NM, 20, 250, 401, 250) == (10, 200)
+    assert T.locate(c, T.GRID_NM, 20, 250, 801, 250) == (10, 400)
+    oc = O.make_config(mx=600, my=600, num_dx=1, den_dx=125)
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        g = int(rng.integers(0, 4))
+        nx, ny = int(rng.integers(0, 1300)), int(rng.integers(0, 1300))
+        den = int(rng.choice([250, 500, 125]))
+        e, ix, iy = O.locate(oc, g, nx, den, ny, den)
+        if e:
+            with pytest.raises(T.TwError):
+                T.locate(c, g, nx, den, ny, den)
+        else:
+            assert T.locate(c, g, nx, den, ny, den) == (ix, iy)
+
+
+def test_interpolation_matches_oracle():
+    import oracle as O
+    c = T.config(mx=40, my=30, num_dx=1, den_dx=100)
+    oc = O.make_config(mx=40, my=30, num_dx=1, den_dx=100)
+    rng = np.random.default_rng(1)
+    model = rng.uniform(1, 2, (21, 16))
+    for g in range(4):
+        np.testing.assert_array_equal(T.interpolate_model(c, g, model, 1, 50), O.interpolate(oc, g, model, 1, 50))
+
+
+def test_ricker_matches_oracle():
+    import oracle as O
+    c = T.config(dt=1e-4, dx=1.0 / 125)
+    oc = O.make_config(dt=1e-4, dx=1.0 / 125)
+    for it in (0, 1, 10, 2499, 2500, 10000):
+        for half in (0, 1):
+            assert T.ricker_increment(c, 5.0, 0.25, it, half) == O.ricker_increment(oc, 5.0, 0.25, it, half)

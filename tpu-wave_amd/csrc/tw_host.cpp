@@ -1,0 +1,986 @@
+// tw_host.cpp -- host side of the C-ABI (include/tw.h).
+//
+// Setup runs in FP64 on the host exactly as the reference builds its grids
+// (cuda/src/cpp/grid.hpp:146-481, grid_parameter.cpp, grid_energy.cpp,
+// forward.cpp); the time loop launches the fused gfx950 kernels of
+// tw_kernels.hip on one HIP stream with no host synchronisation inside
+// tw_step.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tw.h"
+#include "tw_internal.h"
+
+using tw::Plan;
+
+namespace {
+
+// ---- operator constants: cuda/src/cpp/namespace_forward.hpp:65-156 ----
+const double kStencil[4] = {1. / 24, -9. / 8, 9. / 8, -1. / 24};
+const double kDM_L[4][5] = {{-2., 3., -1., 0., 0.},
+                            {-1., 1., 0., 0., 0.},
+                            {1. / 24., -9. / 8., 9. / 8., -1. / 24., 0.},
+                            {-1. / 71., 6. / 71., -83. / 71., 81. / 71., -3. / 71.}};
+const double kDM_R[4][5] = {{3. / 71., -81. / 71., 83. / 71., -6. / 71., 1. / 71.},
+                            {0., 1. / 24., -9. / 8., 9. / 8., -1. / 24.},
+                            {0., 0., 0., -1., 1.},
+                            {0., 0., 1., -3., 2.}};
+const double kDN_L[3][5] = {{-79. / 78., 27. / 26., -1. / 26., 1. / 78., 0.},
+                            {2. / 21., -9. / 7., 9. / 7., -2. / 21., 0.},
+                            {1. / 75., 0., -27. / 25., 83. / 75., -1. / 25.}};
+const double kDN_R[3][5] = {{1. / 25., -83. / 75., 27. / 25., 0., -1. / 75.},
+                            {0., 2. / 21., -9. / 7., 9. / 7., -2. / 21.},
+                            {0., -1. / 78., 1. / 26., -27. / 26., 79. / 78.}};
+const double kAN_L[4] = {7. / 18., 9. / 8., 1., 71. / 72.};
+const double kAN_R[4] = {71. / 72., 1., 9. / 8., 7. / 18.};
+const double kAM_L[3] = {13. / 12., 7. / 8., 25. / 24.};
+const double kAM_R[3] = {25. / 24., 7. / 8., 13. / 12.};
+const double kPM_L[3] = {15. / 8., -5. / 4., 3. / 8.};
+const double kPM_R[3] = {3. / 8., -5. / 4., 15. / 8.};
+
+thread_local std::string g_err;
+
+int fail(std::string* where, int code, const std::string& msg) {
+  if (where) *where = msg;
+  g_err = msg;
+  return code;
+}
+
+int field_grid(int f) {
+  switch (f) {
+    case TW_VX: return TW_GRID_NM;
+    case TW_VY: return TW_GRID_MN;
+    case TW_SXY: return TW_GRID_NN;
+    default: return TW_GRID_MM;
+  }
+}
+int grid_tx(int g) { return g == TW_GRID_NN || g == TW_GRID_NM; }
+int grid_ty(int g) { return g == TW_GRID_NN || g == TW_GRID_MN; }
+int coef_grid(int k) {
+  switch (k) {
+    case TW_C_BINV_VX: return TW_GRID_NM;
+    case TW_C_BINV_VY: return TW_GRID_MN;
+    case TW_C_MU: return TW_GRID_NN;
+    default: return TW_GRID_MM;
+  }
+}
+bool field_in(int physics, int f) {
+  if (f < 0 || f > 5) return false;
+  if (physics == TW_ELASTIC) return f != TW_P;
+  return f == TW_VX || f == TW_VY || f == TW_P;
+}
+bool coef_in(int physics, int k) {
+  if (k < 0 || k > 5) return false;
+  if (physics == TW_ELASTIC) return k != TW_C_KAPPA;
+  return k == TW_C_BINV_VX || k == TW_C_BINV_VY || k == TW_C_KAPPA;
+}
+bool is_velocity(int f) { return f == TW_VX || f == TW_VY; }
+bool ref_precision(int p) { return p == TW_FP64 || p == TW_FP32; }
+bool rmode(int c) { return c == TW_CPST_3R || c == TW_CPST_M3R || c == TW_CPST_6R || c == TW_CPST_K; }
+bool cmode(int c) { return c == TW_CPST_3C || c == TW_CPST_M3C || c == TW_CPST_6C; }
+
+// ---- host conversions (round to nearest even) ----
+uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float bf16_to_f32(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+size_t state_size(int precision) { return precision == TW_FP64 ? 8 : (precision == TW_FP32 ? 4 : 2); }
+size_t coef_size(int precision, int coef16) {
+  if (precision == TW_FP64) return 8;
+  if (precision == TW_FP32) return 4;
+  return coef16 ? 2 : 4;
+}
+size_t arith_size(int precision) { return precision == TW_FP64 ? 8 : 4; }
+
+// double -> storage bytes, and back
+void to_storage(int precision, const double* in, void* out, size_t n) {
+  switch (precision) {
+    case TW_FP64: memcpy(out, in, n * 8); break;
+    case TW_FP32:
+      for (size_t i = 0; i < n; i++) ((float*)out)[i] = (float)in[i];
+      break;
+    case TW_BF16:
+      for (size_t i = 0; i < n; i++) ((uint16_t*)out)[i] = f32_to_bf16((float)in[i]);
+      break;
+    default:
+      for (size_t i = 0; i < n; i++) ((_Float16*)out)[i] = (_Float16)(float)in[i];
+      break;
+  }
+}
+void from_storage(int precision, const void* in, double* out, size_t n) {
+  switch (precision) {
+    case TW_FP64: memcpy(out, in, n * 8); break;
+    case TW_FP32:
+      for (size_t i = 0; i < n; i++) out[i] = ((const float*)in)[i];
+      break;
+    case TW_BF16:
+      for (size_t i = 0; i < n; i++) out[i] = bf16_to_f32(((const uint16_t*)in)[i]);
+      break;
+    default:
+      for (size_t i = 0; i < n; i++) out[i] = (double)(float)((const _Float16*)in)[i];
+      break;
+  }
+}
+// coefficient rounding: its storage type (the reference casts to the solver precision)
+double round_coef(const tw_config* c, double v) {
+  switch (c->precision) {
+    case TW_FP64: return v;
+    case TW_FP32: return (double)(float)v;
+    case TW_BF16: return c->coef16 ? (double)bf16_to_f32(f32_to_bf16((float)v)) : (double)(float)v;
+    default: return c->coef16 ? (double)(float)(_Float16)(float)v : (double)(float)v;
+  }
+}
+int coef_store_precision(const tw_config* c) {
+  if (c->precision == TW_FP64) return TW_FP64;
+  if (c->precision == TW_FP32 || !c->coef16) return TW_FP32;
+  return c->precision;
+}
+// a value rounded to the arithmetic type, returned as double
+double arith(int precision, double v) { return precision == TW_FP64 ? v : (double)(float)v; }
+
+int check_config(const tw_config* c, std::string* err) {
+  if (!c) return fail(err, TW_EINVAL, "null config");
+  if (c->physics != TW_ELASTIC && c->physics != TW_ACOUSTIC) return fail(err, TW_EINVAL, "physics must be TW_ELASTIC or TW_ACOUSTIC");
+  if (c->mx < 8 || c->my < 8) return fail(err, TW_EINVAL, "mx and my must be >= 8");
+  if ((int64_t)c->mx * c->my > (int64_t)1 << 34) return fail(err, TW_ENOMEM, "grid too large");
+  if (!(c->dx > 0) || !(c->dt > 0)) return fail(err, TW_EINVAL, "dx and dt must be > 0");
+  for (int k = 0; k < 4; k++)
+    if (c->bc[k] != 'P' && c->bc[k] != 'F' && c->bc[k] != 'C') return fail(err, TW_EINVAL, "bc must be 'P', 'F' or 'C'");
+  for (int a = 0; a < 2; a++)
+    if ((c->bc[2 * a] == 'P') != (c->bc[2 * a + 1] == 'P')) return fail(err, TW_EINVAL, "periodic sides must pair");
+  if (c->precision < TW_FP64 || c->precision > TW_FP16) return fail(err, TW_EINVAL, "precision");
+  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K) return fail(err, TW_EINVAL, "cpst");
+  if (ref_precision(c->precision) && c->cpst == TW_CPST_K) return fail(err, TW_EINVAL, "cpst K needs 16-bit storage");
+  if (!ref_precision(c->precision) && c->cpst != TW_CPST_0 && c->cpst != TW_CPST_K)
+    return fail(err, TW_EINVAL, "16-bit storage supports cpst 0 or K");
+  if (c->energy_form == 1 && (c->physics != TW_ACOUSTIC || c->bc[0] != 'P' || c->bc[2] != 'P'))
+    return fail(err, TW_EINVAL, "energy_form 1 is the periodic acoustic reference form");
+  if (c->energy_form != 0 && c->energy_form != 1) return fail(err, TW_EINVAL, "energy_form");
+  if (c->nt_max < 0) return fail(err, TW_EINVAL, "nt_max must be >= 0");
+  if (!(c->source_scaling > 0)) return fail(err, TW_EINVAL, "source_scaling must be > 0");
+  return TW_OK;
+}
+
+void shape_of(const tw_config* c, int g, int* rows, int* cols) {
+  *rows = c->mx + grid_tx(g);
+  *cols = c->my + grid_ty(g);
+}
+
+// Operator tables (set_forward_operators, cuda/src/cpp/grid.hpp:274-464):
+// built in FP64, cast to the arithmetic type; SAT lifts P/(dx*A) are cast,
+// then multiplied by dt (itself cast first) and cast again
+// (container_run_time.hpp:124).
+void build_tables(const tw_config* c, tw::Tables* t) {
+  const int P = c->precision;
+  const double dt = c->dt, dx = c->dx;
+  for (int k = 0; k < 4; k++) t->st[k] = arith(P, c->stencil_form ? kStencil[k] * dt / dx : kStencil[k] * (dt / dx));
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 5; j++) {
+      t->DN_L[i][j] = arith(P, kDM_L[i][j] * (dt / dx));
+      t->DN_R[i][j] = arith(P, kDM_R[i][j] * (dt / dx));
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 5; j++) {
+      t->DM_L[i][j] = arith(P, kDN_L[i][j] * (dt / dx));
+      t->DM_R[i][j] = arith(P, kDN_R[i][j] * (dt / dx));
+    }
+  const double dtc = arith(P, dt);
+  t->ainvN_L = arith(P, arith(P, 1.0 / (dx * kAN_L[0])) * dtc);
+  t->ainvN_R = arith(P, arith(P, 1.0 / (dx * kAN_R[3])) * dtc);
+  for (int i = 0; i < 3; i++) {
+    t->ainvM_L[i] = arith(P, arith(P, kPM_L[i] / (dx * kAM_L[i])) * dtc);
+    t->ainvM_R[i] = arith(P, arith(P, kPM_R[i] / (dx * kAM_R[i])) * dtc);
+    t->projM_L[i] = arith(P, kPM_L[i]);
+    t->projM_R[i] = arith(P, kPM_R[i]);
+  }
+  for (int i = 0; i < 4; i++) t->hN[i] = arith(P, kAN_L[i] * dx);
+  for (int i = 0; i < 3; i++) t->hM[i] = arith(P, kAM_L[i] * dx);
+}
+
+// cuda/src/cpp/forward.cpp:30-54
+int locate_1d(int64_t num, int64_t den, int64_t num_dx, int64_t den_dx, int node, int32_t* out) {
+  int64_t n, d;
+  if (node) {
+    n = num * den_dx;
+    d = den * num_dx;
+  } else {
+    n = 2 * num * den_dx - den * num_dx;
+    d = 2 * den * num_dx;
+  }
+  if (d == 0) return TW_EINVAL;
+  if (n % d != 0) return TW_ERANGE;
+  *out = (int32_t)(n / d);
+  return TW_OK;
+}
+
+// cuda/src/cpp/grid_parameter.cpp:6-60 (bilinear, fixed term order)
+int interpolate(const tw_config* c, int g, const double* model, int32_t nxm, int32_t nym, int64_t num_dxm,
+                int64_t den_dxm, double* out) {
+  int rows, cols;
+  shape_of(c, g, &rows, &cols);
+  const int64_t Nx = (int64_t)nxm + 1, Ny = (int64_t)nym + 1;
+  const int64_t nd = c->num_dx, dd = c->den_dx;
+  if (nd <= 0 || dd <= 0) return fail(nullptr, TW_EINVAL, "tw_config num_dx/den_dx must be set for model interpolation");
+  if (num_dxm <= 0 || den_dxm <= 0 || nxm < 1 || nym < 1) return fail(nullptr, TW_EINVAL, "model spacing/size");
+  auto mv = [&](int64_t a, int64_t b) { return (a < Nx && b < Ny) ? model[a * Ny + b] : 0.0; };
+  for (int ix = 0; ix < rows; ix++)
+    for (int iy = 0; iy < cols; iy++) {
+      int64_t nxl, dxl, nyl, dyl;
+      if (grid_tx(g)) { nxl = ix * nd; dxl = dd; } else { nxl = (2 * (int64_t)ix + 1) * nd; dxl = 2 * dd; }
+      if (grid_ty(g)) { nyl = iy * nd; dyl = dd; } else { nyl = (2 * (int64_t)iy + 1) * nd; dyl = 2 * dd; }
+      const int64_t ixm = (nxl * den_dxm) / (dxl * num_dxm);
+      const int64_t iym = (nyl * den_dxm) / (dyl * num_dxm);
+      const double xi = (double)(nxl * den_dxm) / (double)(dxl * num_dxm) - ixm;
+      const double eta = (double)(nyl * den_dxm) / (double)(dyl * num_dxm) - iym;
+      if (ixm < 0 || ixm + 1 > Nx || iym < 0 || iym + 1 > Ny)
+        return fail(nullptr, TW_ERANGE, "model does not cover the simulation grid");
+      const double v = (1 - xi) * (1 - eta) * mv(ixm, iym) + (1 - xi) * (eta)*mv(ixm, iym + 1) +
+                       (xi) * (1 - eta) * mv(ixm + 1, iym) + (xi) * (eta)*mv(ixm + 1, iym + 1);
+      out[(size_t)ix * cols + iy] = v;
+    }
+  return TW_OK;
+}
+
+// forward_simulation.cpp:62-70 / 104-112; device driver divides by source_scaling
+double ricker_inc(const tw_config* c, double f0, double t0, int32_t it, int32_t half) {
+  const double dt = c->dt, dx = c->dx;
+  const double t = half ? (it + 1. / 2.) * (double)dt - t0 : it * (double)dt - t0;
+  const double A = (1 - 2 * M_PI * M_PI * f0 * f0 * t * t) * exp(-M_PI * M_PI * f0 * f0 * t * t);
+  return A * dt / (dx * dx) / c->source_scaling;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- solver
+struct tw_solver {
+  tw_config cfg{};
+  Plan plan{};
+  std::string err;
+  std::vector<double> hcoef[6];
+  std::vector<void*> allocs;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int src_grid = -1, src_ix = 0, src_iy = 0;
+  std::vector<double> src_inc;
+  std::vector<tw::Receiver> rcv_v, rcv_s;
+  int nrcv = 0;
+  int it = 0;
+  bool ep_valid = false;
+  bool need_uhat = false;
+};
+
+namespace {
+
+int hip_fail(tw_solver* s, hipError_t e, const char* what) {
+  return fail(s ? &s->err : nullptr, TW_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(s, call)                                 \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_fail((s), e_, #call); \
+  } while (0)
+
+int dalloc(tw_solver* s, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 256);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(&s->err, TW_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  s->allocs.push_back(*p);
+  e = hipMemset(*p, 0, bytes > 0 ? bytes : 256);
+  if (e != hipSuccess) return hip_fail(s, e, "hipMemset");
+  return TW_OK;
+}
+
+void dfree(tw_solver* s, void** p) {
+  if (!*p) return;
+  (void)hipFree(*p);
+  for (auto& a : s->allocs)
+    if (a == *p) a = nullptr;
+  *p = nullptr;
+}
+
+int64_t pitch_of(int cols) { return ((int64_t)cols + 63) / 64 * 64; }
+
+int upload_coef(tw_solver* s, int k) {
+  const tw_config* c = &s->cfg;
+  tw::DevCoef& d = s->plan.k[k];
+  const int cp = coef_store_precision(c);
+  const size_t es = state_size(cp);
+  std::vector<uint8_t> buf((size_t)d.rows * d.pitch * es, 0);
+  for (int r = 0; r < d.rows; r++)
+    to_storage(cp, s->hcoef[k].data() + (size_t)r * d.cols, buf.data() + (size_t)r * d.pitch * es, (size_t)d.cols);
+  HIPCHK(s, hipMemcpy(d.p, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  return TW_OK;
+}
+
+// impedances Z = sqrt(M / binv) = rho c along characteristic sides
+int upload_z(tw_solver* s) {
+  const tw_config* c = &s->cfg;
+  const bool el = c->physics == TW_ELASTIC;
+  const int Mx = c->mx, My = c->my;
+  for (int side = 0; side < 4; side++) {
+    if (c->bc[side] != 'C') continue;
+    const int axis = side / 2, right = side % 2;
+    for (int f = TW_VX; f <= TW_VY; f++) {
+      const bool normal = (axis == 0) == (f == TW_VX);
+      if (!el && !normal) continue;
+      const tw::DevField& F = s->plan.f[f];
+      const int n = axis == 0 ? F.cols : F.rows;
+      std::vector<double> z(n);
+      const std::vector<double>& binv = s->hcoef[f == TW_VX ? TW_C_BINV_VX : TW_C_BINV_VY];
+      for (int j = 0; j < n; j++) {
+        const int a = axis == 0 ? (right ? F.rows - 1 : 0) : (right ? F.cols - 1 : 0);
+        const size_t qv = axis == 0 ? (size_t)a * F.cols + j : (size_t)j * F.cols + a;
+        double mod;
+        if (normal) {
+          const int am = axis == 0 ? (right ? Mx - 1 : 0) : (right ? My - 1 : 0);
+          const size_t qm = axis == 0 ? (size_t)am * My + j : (size_t)j * My + am;
+          mod = el ? s->hcoef[TW_C_LAMBDA][qm] + s->hcoef[TW_C_TWOMU][qm] : s->hcoef[TW_C_KAPPA][qm];
+        } else {
+          const int an = axis == 0 ? (right ? Mx : 0) : (right ? My : 0);
+          const size_t qn = axis == 0 ? (size_t)an * (My + 1) + j : (size_t)j * (My + 1) + an;
+          mod = s->hcoef[TW_C_MU][qn];
+        }
+        z[j] = sqrt(mod / binv[qv]);
+      }
+      void* dz = s->plan.z[f == TW_VX ? 0 : 1][side];
+      if (c->precision == TW_FP64) {
+        HIPCHK(s, hipMemcpy(dz, z.data(), n * 8, hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> zf(n);
+        for (int j = 0; j < n; j++) zf[j] = (float)z[j];
+        HIPCHK(s, hipMemcpy(dz, zf.data(), n * 4, hipMemcpyHostToDevice));
+      }
+    }
+  }
+  return TW_OK;
+}
+
+int after_coef_change(tw_solver* s) {
+  s->ep_valid = false;
+  return upload_z(s);
+}
+
+void set_default_material(tw_solver* s) {
+  const tw_config* c = &s->cfg;
+  const double rho = 1., vp = c->physics == TW_ELASTIC ? 2. : 1., vs = 1.;
+  for (int k = 0; k < 6; k++) {
+    if (!coef_in(c->physics, k)) continue;
+    double v;
+    switch (k) {
+      case TW_C_BINV_VX:
+      case TW_C_BINV_VY: v = 1. / rho; break;
+      case TW_C_MU: v = rho * vs * vs; break;
+      case TW_C_TWOMU: v = 2 * rho * vs * vs; break;
+      case TW_C_LAMBDA: v = rho * vp * vp - 2 * rho * vs * vs; break;
+      default: v = rho * vp * vp; break;
+    }
+    std::fill(s->hcoef[k].begin(), s->hcoef[k].end(), round_coef(c, v));
+  }
+}
+
+tw::Pending stress_pending(tw_solver* s, int it_done, bool energy) {
+  tw::Pending p;
+  const tw_config* c = &s->cfg;
+  if (energy) {
+    p.part = s->plan.part_s;
+    p.nparts = s->plan.nparts;
+    p.out0 = s->plan.ep1;
+    p.it0 = it_done;
+    if (it_done + 1 < c->nt_max) {
+      p.out1 = s->plan.ep0;
+      p.it1 = it_done + 1;
+    }
+    p.scale = c->energy_form == 1 ? 1.0 : 0.5;
+  }
+  if (it_done < c->nt_max && !s->rcv_s.empty()) {
+    p.rcv = s->plan.rcv_s;
+    p.nrcv = (int32_t)s->rcv_s.size();
+    p.rcv_it = it_done;
+  }
+  return p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C-ABI
+extern "C" {
+
+const char* tw_global_error(void) { return g_err.c_str(); }
+const char* tw_version(void) { return "twave 0.1 (gfx950)"; }
+
+void tw_config_default(tw_config* c) {
+  memset(c, 0, sizeof *c);
+  c->physics = TW_ELASTIC;
+  c->mx = c->my = 64;
+  c->dx = 1.0 / 125;
+  c->num_dx = 1;
+  c->den_dx = 125;
+  c->dt = 1e-4;
+  memcpy(c->bc, "FFPP", 4);
+  c->precision = TW_FP32;
+  c->cpst = TW_CPST_0;
+  c->record_energy = 1;
+  c->nt_max = 1000;
+  c->source_scaling = 1.0;
+}
+
+int tw_config_check(const tw_config* cfg) { return check_config(cfg, nullptr); }
+
+int tw_field_shape(const tw_config* cfg, int32_t field, int32_t* rows, int32_t* cols) {
+  if (!cfg || !rows || !cols) return fail(nullptr, TW_EINVAL, "null argument");
+  if (!field_in(cfg->physics, field)) return fail(nullptr, TW_EINVAL, "field not present in this physics");
+  shape_of(cfg, field_grid(field), rows, cols);
+  return TW_OK;
+}
+
+int tw_locate(const tw_config* cfg, int32_t grid, int64_t num_x, int64_t den_x, int64_t num_y, int64_t den_y,
+              int32_t* ix, int32_t* iy) {
+  if (!cfg || !ix || !iy || grid < 0 || grid > 3) return fail(nullptr, TW_EINVAL, "bad argument");
+  if (cfg->num_dx <= 0 || cfg->den_dx <= 0) return fail(nullptr, TW_EINVAL, "num_dx/den_dx not set");
+  int32_t rx, ry;
+  int e = locate_1d(num_x, den_x, cfg->num_dx, cfg->den_dx, grid_tx(grid), &rx);
+  if (e) return fail(nullptr, e, "location is not a grid point along x");
+  e = locate_1d(num_y, den_y, cfg->num_dx, cfg->den_dx, grid_ty(grid), &ry);
+  if (e) return fail(nullptr, e, "location is not a grid point along y");
+  int rows, cols;
+  shape_of(cfg, grid, &rows, &cols);
+  if (rx < 0 || rx >= rows || ry < 0 || ry >= cols) return fail(nullptr, TW_ERANGE, "location outside the grid");
+  *ix = rx;
+  *iy = ry;
+  return TW_OK;
+}
+
+int tw_interpolate_model(const tw_config* cfg, int32_t grid, const double* model, int32_t nxm, int32_t nym,
+                         int64_t num_dxm, int64_t den_dxm, double* out) {
+  if (!cfg || !model || !out || grid < 0 || grid > 3) return fail(nullptr, TW_EINVAL, "bad argument");
+  return interpolate(cfg, grid, model, nxm, nym, num_dxm, den_dxm, out);
+}
+
+double tw_ricker_increment(const tw_config* cfg, double f0, double t0, int32_t it, int32_t half) {
+  return ricker_inc(cfg, f0, t0, it, half);
+}
+
+int tw_create(const tw_config* cfg, tw_solver** out) {
+  if (!out) return fail(nullptr, TW_EINVAL, "null output handle");
+  *out = nullptr;
+  int e = check_config(cfg, nullptr);
+  if (e) return e;
+  tw_solver* s = new tw_solver();
+  s->cfg = *cfg;
+  Plan& p = s->plan;
+  const tw_config* c = &s->cfg;
+  hipError_t he = hipSetDevice(c->device);
+  if (he != hipSuccess) {
+    hip_fail(s, he, "hipSetDevice");
+    delete s;
+    return TW_EDEVICE;
+  }
+  p.physics = c->physics;
+  p.precision = c->precision;
+  p.coef16 = c->coef16;
+  p.cpst = c->cpst;
+  p.mx = c->mx;
+  p.my = c->my;
+  p.perx = c->bc[0] == 'P';
+  p.pery = c->bc[2] == 'P';
+  memcpy(p.bc, c->bc, 4);
+  p.energy_form = c->energy_form;
+  p.dx = c->dx;
+  build_tables(c, &p.t);
+  const size_t es = state_size(c->precision);
+  auto bail = [&](int code) {
+    tw_destroy(s);
+    return code;
+  };
+  for (int f = 0; f < 6; f++) {
+    if (!field_in(c->physics, f)) continue;
+    tw::DevField& d = p.f[f];
+    shape_of(c, field_grid(f), &d.rows, &d.cols);
+    d.pitch = pitch_of(d.cols);
+    const size_t bytes = (size_t)d.rows * d.pitch * es;
+    if ((e = dalloc(s, &d.s, bytes))) return bail(e);
+    if (rmode(c->cpst) && (e = dalloc(s, &d.r, bytes))) return bail(e);
+    if (cmode(c->cpst) && (e = dalloc(s, &d.c, bytes))) return bail(e);
+  }
+  const size_t ks = coef_size(c->precision, c->coef16);
+  for (int k = 0; k < 6; k++) {
+    if (!coef_in(c->physics, k)) continue;
+    tw::DevCoef& d = p.k[k];
+    shape_of(c, coef_grid(k), &d.rows, &d.cols);
+    d.pitch = pitch_of(d.cols);
+    if ((e = dalloc(s, &d.p, (size_t)d.rows * d.pitch * ks))) return bail(e);
+    s->hcoef[k].assign((size_t)d.rows * d.cols, 0.0);
+  }
+  // characteristic SAT buffers
+  const size_t as = arith_size(c->precision);
+  for (int side = 0; side < 4; side++) {
+    if (c->bc[side] != 'C') continue;
+    const int axis = side / 2;
+    for (int f = TW_VX; f <= TW_VY; f++) {
+      const bool normal = (axis == 0) == (f == TW_VX);
+      if (c->physics == TW_ACOUSTIC && !normal) continue;
+      const tw::DevField& F = p.f[f];
+      const int n = axis == 0 ? F.cols : F.rows;
+      if ((e = dalloc(s, &p.z[f == TW_VX ? 0 : 1][side], (size_t)n * as))) return bail(e);
+      if (!normal) {
+        if ((e = dalloc(s, &p.uhat[f == TW_VX ? 0 : 1][side], (size_t)n * as))) return bail(e);
+        s->need_uhat = true;
+      }
+    }
+  }
+  // records and partial sums
+  p.nt_max = c->nt_max;
+  const size_t nt = (size_t)(c->nt_max > 0 ? c->nt_max : 1);
+  if ((e = dalloc(s, (void**)&p.ek, nt * 8)) || (e = dalloc(s, (void**)&p.ep0, nt * 8)) ||
+      (e = dalloc(s, (void**)&p.ep1, nt * 8)))
+    return bail(e);
+  dim3 g, b;
+  tw::grid_dims(p, &g, &b);
+  p.nparts = (int32_t)(g.x * g.y);
+  if ((e = dalloc(s, (void**)&p.part_v, (size_t)p.nparts * 8)) || (e = dalloc(s, (void**)&p.part_s, (size_t)p.nparts * 8)))
+    return bail(e);
+  he = hipStreamCreate(&s->stream);
+  if (he != hipSuccess) {
+    hip_fail(s, he, "hipStreamCreate");
+    return bail(TW_EDEVICE);
+  }
+  s->own_stream = true;
+  set_default_material(s);
+  for (int k = 0; k < 6; k++)
+    if (coef_in(c->physics, k) && (e = upload_coef(s, k))) return bail(e);
+  if ((e = after_coef_change(s))) return bail(e);
+  *out = s;
+  return TW_OK;
+}
+
+int tw_destroy(tw_solver* s) {
+  if (!s) return TW_OK;
+  (void)hipSetDevice(s->cfg.device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : s->allocs) (void)hipFree(p);
+  if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return TW_OK;
+}
+
+const char* tw_last_error(const tw_solver* s) { return s ? s->err.c_str() : g_err.c_str(); }
+
+int tw_set_model(tw_solver* s, const double* rho, const double* vp, const double* vs, int32_t nxm, int32_t nym,
+                 int64_t num_dxm, int64_t den_dxm) {
+  if (!s || !rho || !vp || (s->cfg.physics == TW_ELASTIC && !vs)) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  (void)hipSetDevice(s->cfg.device);
+  const tw_config* c = &s->cfg;
+  const size_t nm = (size_t)(nxm + 1) * (nym + 1);
+  std::vector<double> a(nm), b(nm);
+  for (int k = 0; k < 6; k++) {
+    if (!coef_in(c->physics, k)) continue;
+    const int g = coef_grid(k);
+    std::vector<double> out(s->hcoef[k].size());
+    int e;
+    switch (k) {
+      case TW_C_BINV_VX:
+      case TW_C_BINV_VY:  // make_density_reciprocal: 1 / (interpolated rho)
+        e = interpolate(c, g, rho, nxm, nym, num_dxm, den_dxm, out.data());
+        for (double& v : out) v = 1. / v;
+        break;
+      case TW_C_MU:
+        for (size_t i = 0; i < nm; i++) a[i] = rho[i] * vs[i] * vs[i];
+        e = interpolate(c, g, a.data(), nxm, nym, num_dxm, den_dxm, out.data());
+        break;
+      case TW_C_TWOMU:
+        for (size_t i = 0; i < nm; i++) a[i] = 2 * rho[i] * vs[i] * vs[i];
+        e = interpolate(c, g, a.data(), nxm, nym, num_dxm, den_dxm, out.data());
+        break;
+      case TW_C_LAMBDA:  // lambda = rho vp^2 - 2 mu on the model nodes
+        for (size_t i = 0; i < nm; i++) {
+          b[i] = 2 * rho[i] * vs[i] * vs[i];
+          a[i] = rho[i] * vp[i] * vp[i] - b[i];
+        }
+        e = interpolate(c, g, a.data(), nxm, nym, num_dxm, den_dxm, out.data());
+        break;
+      default:  // kappa = rho vp^2
+        for (size_t i = 0; i < nm; i++) a[i] = rho[i] * vp[i] * vp[i];
+        e = interpolate(c, g, a.data(), nxm, nym, num_dxm, den_dxm, out.data());
+        break;
+    }
+    if (e) {
+      s->err = g_err;
+      return e;
+    }
+    for (size_t i = 0; i < out.size(); i++) s->hcoef[k][i] = round_coef(c, out[i]);
+    if ((e = upload_coef(s, k))) return e;
+  }
+  return after_coef_change(s);
+}
+
+int tw_set_coef(tw_solver* s, int32_t k, const double* v) {
+  if (!s || !v) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  if (!coef_in(s->cfg.physics, k)) return fail(&s->err, TW_EINVAL, "coefficient not used by this physics");
+  (void)hipSetDevice(s->cfg.device);
+  for (size_t i = 0; i < s->hcoef[k].size(); i++) {
+    if (!(v[i] > 0) && k != TW_C_LAMBDA) return fail(&s->err, TW_EINVAL, "coefficient must be positive");
+    s->hcoef[k][i] = round_coef(&s->cfg, v[i]);
+  }
+  int e = upload_coef(s, k);
+  if (e) return e;
+  return after_coef_change(s);
+}
+
+int tw_get_coef(tw_solver* s, int32_t k, double* v) {
+  if (!s || !v) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  if (!coef_in(s->cfg.physics, k)) return fail(&s->err, TW_EINVAL, "coefficient not used by this physics");
+  memcpy(v, s->hcoef[k].data(), s->hcoef[k].size() * 8);
+  return TW_OK;
+}
+
+static int check_source_point(tw_solver* s, int32_t grid, int32_t ix, int32_t iy) {
+  const tw_config* c = &s->cfg;
+  if (grid < 0 || grid > 3) return fail(&s->err, TW_EINVAL, "grid id");
+  if (c->physics == TW_ACOUSTIC && grid == TW_GRID_NN) return fail(&s->err, TW_EINVAL, "no NN grid in acoustic");
+  int rows, cols;
+  shape_of(c, grid, &rows, &cols);
+  if (ix < 0 || ix >= rows || iy < 0 || iy >= cols) return fail(&s->err, TW_ERANGE, "source outside the grid");
+  // forward.cpp:95-98: no source on a weak (SBP) node boundary
+  if (c->bc[0] != 'P' && grid_tx(grid) && (ix == 0 || ix == rows - 1))
+    return fail(&s->err, TW_ERANGE, "source on a weak boundary along x");
+  if (c->bc[2] != 'P' && grid_ty(grid) && (iy == 0 || iy == cols - 1))
+    return fail(&s->err, TW_ERANGE, "source on a weak boundary along y");
+  return TW_OK;
+}
+
+int tw_set_source_ricker(tw_solver* s, int32_t grid, int32_t ix, int32_t iy, double f0, double t0) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  int e = check_source_point(s, grid, ix, iy);
+  if (e) return e;
+  const int half = !(grid == TW_GRID_MN || grid == TW_GRID_NM);
+  s->src_inc.resize((size_t)s->cfg.nt_max);
+  for (int it = 0; it < s->cfg.nt_max; it++) s->src_inc[it] = ricker_inc(&s->cfg, f0, t0, it, half);
+  s->src_grid = grid;
+  s->src_ix = ix;
+  s->src_iy = iy;
+  return TW_OK;
+}
+
+int tw_set_source_wavelet(tw_solver* s, int32_t grid, int32_t ix, int32_t iy, const double* inc, int32_t n) {
+  if (!s || (!inc && n > 0) || n < 0) return fail(s ? &s->err : nullptr, TW_EINVAL, "bad argument");
+  int e = check_source_point(s, grid, ix, iy);
+  if (e) return e;
+  s->src_inc.assign(inc, inc + n);
+  s->src_grid = grid;
+  s->src_ix = ix;
+  s->src_iy = iy;
+  return TW_OK;
+}
+
+int tw_clear_source(tw_solver* s) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  s->src_grid = -1;
+  s->src_inc.clear();
+  return TW_OK;
+}
+
+int tw_set_receivers(tw_solver* s, int32_t n, const int32_t* field, const int32_t* ix, const int32_t* iy) {
+  if (!s || n < 0 || (n > 0 && (!field || !ix || !iy))) return fail(s ? &s->err : nullptr, TW_EINVAL, "bad argument");
+  (void)hipSetDevice(s->cfg.device);
+  std::vector<tw::Receiver> v, st;
+  for (int r = 0; r < n; r++) {
+    if (!field_in(s->cfg.physics, field[r])) return fail(&s->err, TW_EINVAL, "receiver field not in this physics");
+    int rows, cols;
+    shape_of(&s->cfg, field_grid(field[r]), &rows, &cols);
+    if (ix[r] < 0 || ix[r] >= rows || iy[r] < 0 || iy[r] >= cols) return fail(&s->err, TW_ERANGE, "receiver outside the grid");
+    tw::Receiver rc{field[r], ix[r], iy[r], r};
+    (is_velocity(field[r]) ? v : st).push_back(rc);
+  }
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  // records [n][nt_max]
+  dfree(s, (void**)&s->plan.rec);
+  dfree(s, (void**)&s->plan.rcv_v);
+  dfree(s, (void**)&s->plan.rcv_s);
+  int e;
+  const size_t recn = (size_t)(n > 0 ? n : 1) * (size_t)(s->cfg.nt_max > 0 ? s->cfg.nt_max : 1);
+  if ((e = dalloc(s, (void**)&s->plan.rec, recn * 8))) return e;
+  if ((e = dalloc(s, (void**)&s->plan.rcv_v, (v.size() + 1) * sizeof(tw::Receiver)))) return e;
+  if ((e = dalloc(s, (void**)&s->plan.rcv_s, (st.size() + 1) * sizeof(tw::Receiver)))) return e;
+  if (!v.empty()) HIPCHK(s, hipMemcpy(s->plan.rcv_v, v.data(), v.size() * sizeof(tw::Receiver), hipMemcpyHostToDevice));
+  if (!st.empty()) HIPCHK(s, hipMemcpy(s->plan.rcv_s, st.data(), st.size() * sizeof(tw::Receiver), hipMemcpyHostToDevice));
+  s->rcv_v = v;
+  s->rcv_s = st;
+  s->plan.nrcv_v = (int32_t)v.size();
+  s->plan.nrcv_s = (int32_t)st.size();
+  s->nrcv = n;
+  return TW_OK;
+}
+
+int tw_step(tw_solver* s, int32_t nsteps) {
+  if (!s || nsteps < 0) return fail(s ? &s->err : nullptr, TW_EINVAL, "bad argument");
+  (void)hipSetDevice(s->cfg.device);
+  const tw_config* c = &s->cfg;
+  Plan& p = s->plan;
+  const bool vsrc = s->src_grid == TW_GRID_MN || s->src_grid == TW_GRID_NM;
+  tw::Pending carry;  // stress work of the previous step, done by the next launch
+  bool have_carry = false;
+  for (int k = 0; k < nsteps; k++) {
+    const int it = s->it;
+    const bool rec = c->record_energy && it < c->nt_max;
+    if (rec && !s->ep_valid) {
+      HIPCHK(s, tw::launch_stress_energy(p, s->stream));
+      carry = tw::Pending();
+      carry.part = p.part_s;
+      carry.nparts = p.nparts;
+      carry.out0 = p.ep0;
+      carry.it0 = it;
+      carry.scale = c->energy_form == 1 ? 1.0 : 0.5;
+      have_carry = true;
+      s->ep_valid = true;
+    }
+    if (s->need_uhat) HIPCHK(s, tw::launch_uhat(p, s->stream));
+    tw::StepCtl cv;
+    cv.it = it;
+    cv.energy = rec;
+    if (s->src_grid >= 0 && vsrc && it < (int)s->src_inc.size()) {
+      cv.src_on = 1;
+      cv.src_grid = s->src_grid;
+      cv.src_ix = s->src_ix;
+      cv.src_iy = s->src_iy;
+      cv.src_inc = s->src_inc[it];
+    }
+    if (have_carry) cv.pend = carry;
+    HIPCHK(s, tw::launch_half(p, 0, cv, s->stream));
+    tw::StepCtl cs;
+    cs.it = it;
+    cs.energy = rec;
+    if (s->src_grid >= 0 && !vsrc && it < (int)s->src_inc.size()) {
+      cs.src_on = 1;
+      cs.src_grid = s->src_grid;
+      cs.src_ix = s->src_ix;
+      cs.src_iy = s->src_iy;
+      cs.src_inc = s->src_inc[it];
+    }
+    if (rec) {
+      cs.pend.part = p.part_v;
+      cs.pend.nparts = p.nparts;
+      cs.pend.out0 = p.ek;
+      cs.pend.it0 = it;
+      cs.pend.scale = c->energy_form == 1 ? 1.0 : 0.5;
+    }
+    if (it < c->nt_max && !s->rcv_v.empty()) {
+      cs.pend.rcv = p.rcv_v;
+      cs.pend.nrcv = (int32_t)s->rcv_v.size();
+      cs.pend.rcv_it = it;
+    }
+    HIPCHK(s, tw::launch_half(p, 1, cs, s->stream));
+    carry = stress_pending(s, it, rec);
+    have_carry = true;
+    s->it++;
+  }
+  if (have_carry) HIPCHK(s, tw::launch_finalize(p, carry, s->stream));
+  return TW_OK;
+}
+
+int tw_synchronize(tw_solver* s) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  (void)hipSetDevice(s->cfg.device);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return TW_OK;
+}
+
+int tw_current_step(tw_solver* s, int32_t* it) {
+  if (!s || !it) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  *it = s->it;
+  return TW_OK;
+}
+
+int tw_read_receivers(tw_solver* s, double* out, int32_t* nt) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  (void)hipSetDevice(s->cfg.device);
+  const int n = s->it < s->cfg.nt_max ? s->it : s->cfg.nt_max;
+  if (nt) *nt = n;
+  if (!out || s->nrcv == 0) return TW_OK;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  std::vector<double> buf((size_t)s->nrcv * s->cfg.nt_max);
+  HIPCHK(s, hipMemcpy(buf.data(), s->plan.rec, buf.size() * 8, hipMemcpyDeviceToHost));
+  for (int r = 0; r < s->nrcv; r++)
+    for (int t = 0; t < n; t++) out[(size_t)r * n + t] = s->cfg.source_scaling * buf[(size_t)r * s->cfg.nt_max + t];
+  return TW_OK;
+}
+
+int tw_read_energy(tw_solver* s, double* ek, double* ep0, double* ep1, int32_t* nt) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  (void)hipSetDevice(s->cfg.device);
+  const int n = s->it < s->cfg.nt_max ? s->it : s->cfg.nt_max;
+  if (nt) *nt = n;
+  if (n == 0) return TW_OK;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const double s2 = s->cfg.source_scaling * s->cfg.source_scaling;
+  std::vector<double> b(n);
+  double* outs[3] = {ek, ep0, ep1};
+  double* devs[3] = {s->plan.ek, s->plan.ep0, s->plan.ep1};
+  for (int i = 0; i < 3; i++) {
+    if (!outs[i]) continue;
+    HIPCHK(s, hipMemcpy(b.data(), devs[i], (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (int t = 0; t < n; t++) outs[i][t] = s2 * b[t];
+  }
+  return TW_OK;
+}
+
+static int copy_field(tw_solver* s, int32_t field, int which, double* out, const double* in) {
+  if (!s || (!out && !in)) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  if (!field_in(s->cfg.physics, field)) return fail(&s->err, TW_EINVAL, "field not present in this physics");
+  (void)hipSetDevice(s->cfg.device);
+  const tw::DevField& d = s->plan.f[field];
+  void* dev = which == 0 ? d.s : (which == 1 ? d.r : d.c);
+  const size_t es = state_size(s->cfg.precision);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (!dev) {
+    if (out) memset(out, 0, (size_t)d.rows * d.cols * 8);
+    return in ? fail(&s->err, TW_ESTATE, "no companion array in this compensation mode") : TW_OK;
+  }
+  std::vector<uint8_t> buf((size_t)d.rows * d.cols * es);
+  if (out) {
+    HIPCHK(s, hipMemcpy2D(buf.data(), d.cols * es, dev, d.pitch * es, d.cols * es, d.rows, hipMemcpyDeviceToHost));
+    from_storage(s->cfg.precision, buf.data(), out, (size_t)d.rows * d.cols);
+  } else {
+    to_storage(s->cfg.precision, in, buf.data(), (size_t)d.rows * d.cols);
+    HIPCHK(s, hipMemcpy2D(dev, d.pitch * es, buf.data(), d.cols * es, d.cols * es, d.rows, hipMemcpyHostToDevice));
+    s->ep_valid = false;
+  }
+  return TW_OK;
+}
+
+int tw_get_field(tw_solver* s, int32_t field, double* out) { return copy_field(s, field, 0, out, nullptr); }
+int tw_set_field(tw_solver* s, int32_t field, const double* in) { return copy_field(s, field, 0, nullptr, in); }
+int tw_get_companion(tw_solver* s, int32_t field, double* out) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  return copy_field(s, field, cmode(s->cfg.cpst) ? 2 : 1, out, nullptr);
+}
+
+int tw_reset(tw_solver* s) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  (void)hipSetDevice(s->cfg.device);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const size_t es = state_size(s->cfg.precision);
+  for (int f = 0; f < 6; f++) {
+    const tw::DevField& d = s->plan.f[f];
+    const size_t bytes = (size_t)d.rows * d.pitch * es;
+    for (void* p : {d.s, d.r, d.c})
+      if (p) HIPCHK(s, hipMemset(p, 0, bytes));
+  }
+  const size_t nt = (size_t)(s->cfg.nt_max > 0 ? s->cfg.nt_max : 1);
+  for (double* p : {s->plan.ek, s->plan.ep0, s->plan.ep1}) HIPCHK(s, hipMemset(p, 0, nt * 8));
+  if (s->plan.rec) HIPCHK(s, hipMemset(s->plan.rec, 0, (size_t)(s->nrcv > 0 ? s->nrcv : 1) * nt * 8));
+  s->it = 0;
+  s->ep_valid = false;
+  return TW_OK;
+}
+
+// checkpoint layout: magic, version, step, then for each present array of
+// each field (s, r, c) its rows*cols storage elements.
+static const uint64_t kMagic = 0x3158414d56415754ull;  // "TWAVMAX1"
+
+int tw_checkpoint_size(tw_solver* s, size_t* bytes) {
+  if (!s || !bytes) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  size_t n = 3 * sizeof(uint64_t);
+  const size_t es = state_size(s->cfg.precision);
+  for (int f = 0; f < 6; f++) {
+    const tw::DevField& d = s->plan.f[f];
+    for (void* p : {d.s, d.r, d.c})
+      if (p) n += (size_t)d.rows * d.cols * es;
+  }
+  *bytes = n;
+  return TW_OK;
+}
+
+int tw_checkpoint_save(tw_solver* s, void* buf, size_t bytes) {
+  size_t need;
+  int e = tw_checkpoint_size(s, &need);
+  if (e) return e;
+  if (!buf || bytes < need) return fail(&s->err, TW_EINVAL, "checkpoint buffer too small");
+  (void)hipSetDevice(s->cfg.device);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  uint64_t* h = (uint64_t*)buf;
+  h[0] = kMagic;
+  h[1] = ((uint64_t)s->cfg.precision << 32) | (uint64_t)s->cfg.cpst;
+  h[2] = (uint64_t)s->it;
+  uint8_t* o = (uint8_t*)buf + 3 * sizeof(uint64_t);
+  const size_t es = state_size(s->cfg.precision);
+  for (int f = 0; f < 6; f++) {
+    const tw::DevField& d = s->plan.f[f];
+    for (void* p : {d.s, d.r, d.c}) {
+      if (!p) continue;
+      HIPCHK(s, hipMemcpy2D(o, d.cols * es, p, d.pitch * es, d.cols * es, d.rows, hipMemcpyDeviceToHost));
+      o += (size_t)d.rows * d.cols * es;
+    }
+  }
+  return TW_OK;
+}
+
+int tw_checkpoint_load(tw_solver* s, const void* buf, size_t bytes) {
+  size_t need;
+  int e = tw_checkpoint_size(s, &need);
+  if (e) return e;
+  if (!buf || bytes < need) return fail(&s->err, TW_EINVAL, "checkpoint buffer too small");
+  const uint64_t* h = (const uint64_t*)buf;
+  if (h[0] != kMagic || h[1] != (((uint64_t)s->cfg.precision << 32) | (uint64_t)s->cfg.cpst))
+    return fail(&s->err, TW_EINVAL, "checkpoint does not match this solver");
+  (void)hipSetDevice(s->cfg.device);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const uint8_t* o = (const uint8_t*)buf + 3 * sizeof(uint64_t);
+  const size_t es = state_size(s->cfg.precision);
+  for (int f = 0; f < 6; f++) {
+    const tw::DevField& d = s->plan.f[f];
+    for (void* p : {d.s, d.r, d.c}) {
+      if (!p) continue;
+      HIPCHK(s, hipMemcpy2D(p, d.pitch * es, o, d.cols * es, d.cols * es, d.rows, hipMemcpyHostToDevice));
+      o += (size_t)d.rows * d.cols * es;
+    }
+  }
+  s->it = (int)h[2];
+  s->ep_valid = false;
+  return TW_OK;
+}
+
+int tw_set_stream(tw_solver* s, void* stream) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  (void)hipSetDevice(s->cfg.device);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (s->own_stream) (void)hipStreamDestroy(s->stream);
+  s->stream = (hipStream_t)stream;
+  s->own_stream = false;
+  return TW_OK;
+}
+
+int tw_check_finite(tw_solver* s) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  (void)hipSetDevice(s->cfg.device);
+  int32_t* flag = nullptr;
+  HIPCHK(s, hipMalloc((void**)&flag, sizeof(int32_t)));
+  HIPCHK(s, hipMemsetAsync(flag, 0, sizeof(int32_t), s->stream));
+  hipError_t e = tw::launch_check_finite(s->plan, flag, s->stream);
+  int32_t h = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof h, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(flag);
+  if (e != hipSuccess) return hip_fail(s, e, "check_finite");
+  if (h) return fail(&s->err, TW_ENONFINITE, "non-finite value in the state");
+  return TW_OK;
+}
+
+}  // extern "C"

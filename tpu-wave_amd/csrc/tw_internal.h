@@ -1,0 +1,110 @@
+// tw_internal.h -- plan shared by the host library (tw_host.cpp) and the
+// gfx950 kernels (tw_kernels.hip).  Plain pointers and sizes only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tw.h"
+
+namespace tw {
+
+enum { NF = 6 };
+
+// One field in HBM: row-major [rows][pitch] elements of the storage type.
+struct DevField {
+  void* s = nullptr;  // state
+  void* r = nullptr;  // R companion (3R/-3R/6R residual, or the 16-bit Kahan residual)
+  void* c = nullptr;  // C companion (3C/-3C/6C)
+  int32_t rows = 0, cols = 0;
+  int64_t pitch = 0;  // elements
+};
+
+struct DevCoef {
+  void* p = nullptr;  // coefficient storage type (FP64, FP32, or 16-bit)
+  int32_t rows = 0, cols = 0;
+  int64_t pitch = 0;
+};
+
+// Operator tables, already rounded to the arithmetic type and held as double
+// (exactly representable); the kernel side converts them once per launch.
+struct Tables {
+  double st[4];
+  double DN_L[4][5], DN_R[4][5];  // N-type output (partner M): W_intr.DM_*
+  double DM_L[3][5], DM_R[3][5];  // M-type output (partner N): W_intr.DN_*
+  double ainvN_L, ainvN_R;
+  double ainvM_L[3], ainvM_R[3];
+  double projM_L[3], projM_R[3];
+  double hN[4], hM[3];
+};
+
+struct Receiver {
+  int32_t field, ix, iy, row;  // row: index into the record array
+};
+
+// Work the first block of a launch does for the previous launch: reduce its
+// energy partials in a fixed order and gather the receivers of the fields it
+// finished (those fields are not written by the current launch).
+struct Pending {
+  const double* part = nullptr;
+  int32_t nparts = 0;
+  double* out0 = nullptr;  // out0[it0] = sum * scale
+  int32_t it0 = 0;
+  double* out1 = nullptr;  // out1[it1] = same value (stress energy carried to E_p0)
+  int32_t it1 = 0;
+  double scale = 1.0;       // 1/2 for the H-norm form (grid_energy.cpp:215)
+  const Receiver* rcv = nullptr;
+  int32_t nrcv = 0;
+  int32_t rcv_it = 0;
+};
+
+struct Plan {
+  int32_t physics;
+  int32_t precision;
+  int32_t coef16;
+  int32_t cpst;
+  int32_t mx, my;
+  int32_t perx, pery;
+  char bc[4];
+  int32_t energy_form;
+  double dx;
+  Tables t;
+  DevField f[NF];
+  DevCoef k[NF];
+  // characteristic SAT: impedance per side, arithmetic type, for VX (0) and VY (1)
+  void* z[2][4];
+  // boundary extrapolation of M-type velocities (vy on x sides, vx on y sides)
+  void* uhat[2][4];
+  // records
+  double* rec = nullptr;  // [nrcv][nt_max]
+  int32_t nt_max = 0;
+  double* ek = nullptr;
+  double* ep0 = nullptr;
+  double* ep1 = nullptr;
+  double* part_v = nullptr;  // energy partials of the velocity launch
+  double* part_s = nullptr;  // energy partials of the stress launch
+  int32_t nparts = 0;
+  Receiver* rcv_v = nullptr;  // receivers on velocity fields
+  int32_t nrcv_v = 0;
+  Receiver* rcv_s = nullptr;  // receivers on stress / pressure fields
+  int32_t nrcv_s = 0;
+};
+
+struct StepCtl {
+  int32_t it = 0;
+  int32_t energy = 0;  // record energy this step
+  // source
+  int32_t src_grid = -1, src_ix = 0, src_iy = 0;
+  double src_inc = 0.0;
+  int32_t src_on = 0;
+  Pending pend;
+};
+
+// launches (tw_kernels.hip)
+void grid_dims(const Plan& p, dim3* grid, dim3* block);
+hipError_t launch_half(const Plan& p, int half, const StepCtl& c, hipStream_t s);
+hipError_t launch_uhat(const Plan& p, hipStream_t s);
+hipError_t launch_stress_energy(const Plan& p, hipStream_t s);
+hipError_t launch_finalize(const Plan& p, const Pending& pend, hipStream_t s);
+hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s);
+
+}  // namespace tw

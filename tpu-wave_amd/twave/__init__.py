@@ -1,0 +1,321 @@
+"""twave -- Python binding of the MI355X staggered SBP-SAT wave solver.
+
+A thin ctypes layer over the C-ABI in include/tw.h (libtwave.so, built for
+gfx950).  It mirrors the reference solver's entry points: grid/material
+setup, source and receivers, time stepping, receiver and energy readout.
+There is no CPU fallback: every call runs the HIP library, and a missing
+library is an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtwave.so")
+
+ELASTIC, ACOUSTIC = 0, 1
+FP64, FP32, BF16, FP16 = 0, 1, 2, 3
+PRECISION = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "fp16": FP16}
+CPST = {"0": 0, "3R": 1, "-3R": 2, "6R": 3, "3C": 4, "-3C": 5, "6C": 6, "K": 7}
+VX, VY, SXX, SYY, SXY, P = 0, 1, 2, 3, 4, 5
+FIELD_NAMES = {VX: "vx", VY: "vy", SXX: "sxx", SYY: "syy", SXY: "sxy", P: "p"}
+GRID_NN, GRID_MN, GRID_NM, GRID_MM = 0, 1, 2, 3
+C_BINV_VX, C_BINV_VY, C_MU, C_LAMBDA, C_TWOMU, C_KAPPA = 0, 1, 2, 3, 4, 5
+
+ERRORS = {1: "TW_EINVAL", 2: "TW_ERANGE", 3: "TW_ENOMEM", 4: "TW_EDEVICE", 5: "TW_ENONFINITE", 6: "TW_ESTATE"}
+
+
+class TwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, str(code)), msg))
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    """tw_config (include/tw.h)."""
+    _fields_ = [
+        ("physics", ctypes.c_int32),
+        ("mx", ctypes.c_int32),
+        ("my", ctypes.c_int32),
+        ("dx", ctypes.c_double),
+        ("num_dx", ctypes.c_int64),
+        ("den_dx", ctypes.c_int64),
+        ("dt", ctypes.c_double),
+        ("bc", ctypes.c_char * 4),
+        ("precision", ctypes.c_int32),
+        ("coef16", ctypes.c_int32),
+        ("cpst", ctypes.c_int32),
+        ("stencil_form", ctypes.c_int32),
+        ("energy_form", ctypes.c_int32),
+        ("record_energy", ctypes.c_int32),
+        ("nt_max", ctypes.c_int32),
+        ("source_scaling", ctypes.c_double),
+        ("device", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+# (name, restype, argtypes)
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_cp = ctypes.POINTER(Config)
+_SIGS = [
+    ("tw_config_default", None, [_cp]),
+    ("tw_config_check", ctypes.c_int, [_cp]),
+    ("tw_field_shape", ctypes.c_int, [_cp, ctypes.c_int32, _ip, _ip]),
+    ("tw_locate", ctypes.c_int, [_cp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                 ctypes.c_int64, _ip, _ip]),
+    ("tw_interpolate_model", ctypes.c_int, [_cp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int64, ctypes.c_int64, _dp]),
+    ("tw_ricker_increment", ctypes.c_double, [_cp, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                              ctypes.c_int32]),
+    ("tw_global_error", ctypes.c_char_p, []),
+    ("tw_version", ctypes.c_char_p, []),
+    ("tw_create", ctypes.c_int, [_cp, ctypes.POINTER(_vp)]),
+    ("tw_destroy", ctypes.c_int, [_vp]),
+    ("tw_last_error", ctypes.c_char_p, [_vp]),
+    ("tw_set_model", ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                    ctypes.c_int64]),
+    ("tw_set_coef", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    ("tw_get_coef", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    ("tw_set_source_ricker", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_double, ctypes.c_double]),
+    ("tw_set_source_wavelet", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp,
+                                             ctypes.c_int32]),
+    ("tw_clear_source", ctypes.c_int, [_vp]),
+    ("tw_set_receivers", ctypes.c_int, [_vp, ctypes.c_int32, _ip, _ip, _ip]),
+    ("tw_step", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("tw_synchronize", ctypes.c_int, [_vp]),
+    ("tw_current_step", ctypes.c_int, [_vp, _ip]),
+    ("tw_read_receivers", ctypes.c_int, [_vp, _dp, _ip]),
+    ("tw_read_energy", ctypes.c_int, [_vp, _dp, _dp, _dp, _ip]),
+    ("tw_get_field", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    ("tw_set_field", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    ("tw_get_companion", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    ("tw_reset", ctypes.c_int, [_vp]),
+    ("tw_checkpoint_size", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    ("tw_checkpoint_save", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    ("tw_checkpoint_load", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    ("tw_set_stream", ctypes.c_int, [_vp, _vp]),
+    ("tw_check_finite", ctypes.c_int, [_vp]),
+]
+
+
+def lib():
+    """Load libtwave.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtwave.so not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return [s[0] for s in _SIGS]
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def config(physics=ELASTIC, mx=64, my=64, dx=1.0 / 125, num_dx=1, den_dx=125, dt=1e-4, bc="FFPP",
+           precision=FP32, coef16=0, cpst="0", stencil_form=0, energy_form=0, record_energy=1, nt_max=1000,
+           source_scaling=1.0, device=0):
+    c = Config()
+    c.physics = physics
+    c.mx, c.my = mx, my
+    c.dx = dx
+    c.num_dx, c.den_dx = num_dx, den_dx
+    c.dt = dt
+    c.bc = bc.encode() if isinstance(bc, str) else bc
+    c.precision = PRECISION[precision] if isinstance(precision, str) else precision
+    c.coef16 = coef16
+    c.cpst = CPST[cpst] if isinstance(cpst, str) else cpst
+    c.stencil_form = stencil_form
+    c.energy_form = energy_form
+    c.record_energy = record_energy
+    c.nt_max = nt_max
+    c.source_scaling = source_scaling
+    c.device = device
+    return c
+
+
+def check_config(cfg):
+    e = lib().tw_config_check(ctypes.byref(cfg))
+    if e:
+        raise TwError(e, lib().tw_global_error().decode())
+
+
+def field_shape(cfg, field):
+    r, c = ctypes.c_int32(), ctypes.c_int32()
+    e = lib().tw_field_shape(ctypes.byref(cfg), field, ctypes.byref(r), ctypes.byref(c))
+    if e:
+        raise TwError(e, lib().tw_global_error().decode())
+    return r.value, c.value
+
+
+def locate(cfg, grid, num_x, den_x, num_y, den_y):
+    """Rational physical location -> (ix, iy) on a grid (forward.cpp:30-54)."""
+    ix, iy = ctypes.c_int32(), ctypes.c_int32()
+    e = lib().tw_locate(ctypes.byref(cfg), grid, num_x, den_x, num_y, den_y, ctypes.byref(ix), ctypes.byref(iy))
+    if e:
+        raise TwError(e, lib().tw_global_error().decode())
+    return ix.value, iy.value
+
+
+def interpolate_model(cfg, grid, model, num_dxm, den_dxm):
+    model = np.ascontiguousarray(model, dtype=np.float64)
+    rows = cfg.mx + (1 if grid in (GRID_NN, GRID_NM) else 0)
+    cols = cfg.my + (1 if grid in (GRID_NN, GRID_MN) else 0)
+    out = np.zeros((rows, cols))
+    e = lib().tw_interpolate_model(ctypes.byref(cfg), grid, _d(model), model.shape[0] - 1, model.shape[1] - 1,
+                                   num_dxm, den_dxm, _d(out))
+    if e:
+        raise TwError(e, lib().tw_global_error().decode())
+    return out
+
+
+def ricker_increment(cfg, f0, t0, it, half):
+    return lib().tw_ricker_increment(ctypes.byref(cfg), f0, t0, it, half)
+
+
+class Solver:
+    """One solver handle (tw_create ... tw_destroy) on a HIP device."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        e = lib().tw_create(ctypes.byref(cfg), ctypes.byref(h))
+        if e:
+            raise TwError(e, lib().tw_global_error().decode())
+        self.h = h
+        self.nrcv = 0
+        self.fields = [VX, VY, SXX, SYY, SXY] if cfg.physics == ELASTIC else [VX, VY, P]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().tw_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, e):
+        if e:
+            raise TwError(e, lib().tw_last_error(self.h).decode())
+
+    def shape(self, field):
+        return field_shape(self.cfg, field)
+
+    def set_model(self, rho, vp, vs, num_dxm, den_dxm):
+        rho, vp, vs = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vp, vs))
+        self._chk(lib().tw_set_model(self.h, _d(rho), _d(vp), _d(vs), rho.shape[0] - 1, rho.shape[1] - 1,
+                                     num_dxm, den_dxm))
+
+    def set_coef(self, k, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._chk(lib().tw_set_coef(self.h, k, _d(v)))
+
+    def get_coef(self, k):
+        grid = {C_BINV_VX: GRID_NM, C_BINV_VY: GRID_MN, C_MU: GRID_NN}.get(k, GRID_MM)
+        rows = self.cfg.mx + (1 if grid in (GRID_NN, GRID_NM) else 0)
+        cols = self.cfg.my + (1 if grid in (GRID_NN, GRID_MN) else 0)
+        out = np.zeros((rows, cols))
+        self._chk(lib().tw_get_coef(self.h, k, _d(out)))
+        return out
+
+    def set_source_ricker(self, grid, ix, iy, f0, t0):
+        self._chk(lib().tw_set_source_ricker(self.h, grid, ix, iy, f0, t0))
+
+    def set_source(self, grid, ix, iy, increments):
+        inc = np.ascontiguousarray(increments, dtype=np.float64)
+        self._chk(lib().tw_set_source_wavelet(self.h, grid, ix, iy, _d(inc), len(inc)))
+
+    def clear_source(self):
+        self._chk(lib().tw_clear_source(self.h))
+
+    def set_receivers(self, fields, ixs, iys):
+        f = np.ascontiguousarray(fields, dtype=np.int32)
+        a = np.ascontiguousarray(ixs, dtype=np.int32)
+        b = np.ascontiguousarray(iys, dtype=np.int32)
+        self._chk(lib().tw_set_receivers(self.h, len(f), _i(f), _i(a), _i(b)))
+        self.nrcv = len(f)
+
+    def step(self, n):
+        self._chk(lib().tw_step(self.h, n))
+
+    def synchronize(self):
+        self._chk(lib().tw_synchronize(self.h))
+
+    def current_step(self):
+        it = ctypes.c_int32()
+        self._chk(lib().tw_current_step(self.h, ctypes.byref(it)))
+        return it.value
+
+    def receivers(self):
+        nt = ctypes.c_int32()
+        self._chk(lib().tw_read_receivers(self.h, None, ctypes.byref(nt)))
+        out = np.zeros((self.nrcv, nt.value))
+        if self.nrcv:
+            self._chk(lib().tw_read_receivers(self.h, _d(out), ctypes.byref(nt)))
+        return out
+
+    def energy(self):
+        nt = ctypes.c_int32()
+        self._chk(lib().tw_read_energy(self.h, None, None, None, ctypes.byref(nt)))
+        ek, ep0, ep1 = (np.zeros(nt.value) for _ in range(3))
+        self._chk(lib().tw_read_energy(self.h, _d(ek), _d(ep0), _d(ep1), ctypes.byref(nt)))
+        return ek, ep0, ep1
+
+    def get_field(self, field):
+        out = np.zeros(self.shape(field))
+        self._chk(lib().tw_get_field(self.h, field, _d(out)))
+        return out
+
+    def set_field(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != self.shape(field):
+            raise ValueError("field shape %s, expected %s" % (v.shape, self.shape(field)))
+        self._chk(lib().tw_set_field(self.h, field, _d(v)))
+
+    def get_companion(self, field):
+        out = np.zeros(self.shape(field))
+        self._chk(lib().tw_get_companion(self.h, field, _d(out)))
+        return out
+
+    def reset(self):
+        self._chk(lib().tw_reset(self.h))
+
+    def checkpoint(self):
+        n = ctypes.c_size_t()
+        self._chk(lib().tw_checkpoint_size(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._chk(lib().tw_checkpoint_save(self.h, ctypes.cast(buf, ctypes.c_void_p), n.value))
+        return buf.raw
+
+    def restore(self, blob):
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        self._chk(lib().tw_checkpoint_load(self.h, ctypes.cast(buf, ctypes.c_void_p), len(blob)))
+
+    def set_stream(self, stream_ptr):
+        self._chk(lib().tw_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def check_finite(self):
+        e = lib().tw_check_finite(self.h)
+        return e == 0 if e in (0, 5) else self._chk(e)
