@@ -1,0 +1,232 @@
+"""Benchmark of the elastic time-step hot path on one MI355X.
+
+Workload (the mixed-precision elastic configuration): 8192x8192 cells,
+dx = 5 m, seeded 20-layer model (Vp 1.5-4 km/s per layer, 5% correlated
+perturbation, Vs = Vp/1.73, rho = 310 Vp^0.25), 15 Hz Ricker vertical force at
+100 m depth, free-surface SAT on top, characteristic SAT on the other sides,
+256 surface receivers.  Fields stored in BF16 with FP32 arithmetic.
+
+A step is one full leapfrog step (velocity and stress half steps, SAT,
+compensated update, source, receivers, FP64 energy) of all 5 fields.
+value = cell updates per second over the K timed steps, inputs resident in
+HBM.  The same run times the FP32-storage solver on the same workload,
+measures the BF16 error against an FP64 run of the same inputs, and times
+the CPU oracle on a bounded sample of the workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size M]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tpu-wave_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "grid-pt updates/s & % HBM peak; BF16-store vs FP32 at ≤1e-4 rel L2 vs FP64"
+
+
+def algorithmic_bytes(precision, cpst, coef16, physics=0):
+    """Compulsory HBM bytes per cell update: 2 F s (state r+w) + 2 F c
+    (stored compensation r+w) + K p (coefficients read)."""
+    F, K = (5, 5) if physics == 0 else (3, 3)
+    s = {0: 8, 1: 4, 2: 2, 3: 2}[precision]
+    comp = cpst in ("3R", "-3R", "6R", "3C", "-3C", "6C", "K")
+    c = s if comp else 0
+    p = 8 if precision == 0 else (2 if coef16 else 4)
+    return 2 * F * s + 2 * F * c + K * p
+
+
+def kernel_source_hash():
+    """SHA-256 (first 16 hex digits) of the HIP sources: a measured traffic
+    figure is used only for the kernels it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "tpu-wave_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".h", ".cpp")):
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(key):
+    """HBM bytes per step measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    calibrated by a copy kernel), from profiles/traffic.txt lines
+    'traffic <workload-key> <kernel-hash> <bytes-per-step>'; None if absent
+    or measured on other kernel sources."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kh = kernel_source_hash()
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) == 4 and t[0] == "traffic" and t[1] == key and t[2] == kh:
+                return float(t[3])
+    return None
+
+
+def copy_calibration(torch, nbytes, reps):
+    """Plain device copy of nbytes, reps times: the known-size kernel that
+    scales the FETCH_SIZE / WRITE_SIZE counters."""
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+
+
+def make_solver(T, W, m, precision, cpst, coef16, nt, model_from=None):
+    cfg = W.elastic_config(m=m, precision=precision, cpst=cpst, coef16=coef16, nt=nt)
+    s = T.Solver(cfg)
+    W.setup_elastic(s, m, 0.005)
+    if model_from is not None:  # same (quantized) coefficients as the low-precision run
+        for k in (T.C_BINV_VX, T.C_BINV_VY, T.C_MU, T.C_LAMBDA, T.C_TWOMU):
+            s.set_coef(k, model_from.get_coef(k))
+    return s
+
+
+def timed_steps(torch, s, steps, warmup):
+    stream = torch.cuda.Stream()
+    s.set_stream(stream.cuda_stream)
+    if warmup:
+        s.step(warmup)
+    s.synchronize()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    s.step(steps)
+    e1.record(stream)
+    s.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / 1e3, wall
+
+
+def cpu_baseline(m, seconds_target=15.0):
+    """Oracle (FP32, all host threads) on the same workload, bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import twave as T
+    import twave.workloads as W
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    cfg = W.elastic_config(m=m, precision=T.FP32, nt=8)
+    oc = O.make_config(physics=0, mx=m, my=m, dx=cfg.dx, num_dx=cfg.num_dx, den_dx=cfg.den_dx, dt=cfg.dt,
+                       bc="FCCC", precision=O.FP32, nt_max=8)
+    o = O.Oracle(oc)
+    rho, vp, vs = W.layered_elastic_model(m, 0.005)
+    o.set_model(rho, vp, vs, 1, 200)
+    o.set_source_ricker(O.GRID_NM, 20, m // 2, 15.0, 1.25 / 15.0)
+    o.step(1)
+    n, t = 0, 0.0
+    while t < seconds_target and n < 7:
+        t0 = time.perf_counter()
+        o.step(1)
+        t += time.perf_counter() - t0
+        n += 1
+    return {"value": m * m * n / t, "unit": "grid-point updates/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (oracle/tw_oracle.c, FP32, OpenMP) on the same %dx%d elastic workload, "
+                      "%d steps (%.1f s) incl. per-step FP64 energy" % (m, m, n, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=8192)
+    ap.add_argument("--precision", default="bf16")
+    ap.add_argument("--cpst", default="0")
+    ap.add_argument("--coef16", type=int, default=0)
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP32 and FP64 runs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-size", type=int, default=0, help="grid of the CPU sample (default: --size)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (the slab decomposition "
+              "across GPUs is a design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if args.copy_calib:
+        copy_calibration(torch, args.copy_calib, 5)
+        return
+    import twave as T
+    import twave.workloads as W
+    torch.cuda.set_device(0)
+    m, K, Wn = args.size, args.steps, args.warmup
+    nt = K + Wn
+    prec = T.PRECISION[args.precision]
+
+    s = make_solver(T, W, m, prec, args.cpst, args.coef16, nt)
+    t_gpu, wall = timed_steps(torch, s, K, Wn)
+    value = m * m * K / t_gpu
+    B = algorithmic_bytes(prec, args.cpst, args.coef16)
+    achieved = B * m * m / (t_gpu / K) / 1e9
+    key = "elastic-%d-%s-%s-%d" % (m, args.precision, args.cpst, args.coef16)
+    traffic = measured_traffic(key)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-point updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": Wn,
+        "ms_per_step": t_gpu / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec != T.FP64 else "f64",
+        "data": "synthetic",
+        "config": {"workload": "elastic %dx%d, dx=5 m, 20-layer model, Ricker 15 Hz at 100 m, FCCC SAT, "
+                               "256 receivers; storage %s, cpst %s, coef16 %d" %
+                               (m, m, args.precision, args.cpst, args.coef16),
+                   "cells": m * m, "fields": 5, "storage": args.precision, "cpst": args.cpst,
+                   "coef16": args.coef16},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "one step = velocity + stress half-step kernels",
+                     "bytes_per_update": B, "traffic_unit": "HBM bytes per step",
+                     "traffic_key": key, "kernel_hash": kernel_source_hash()},
+    }
+    if not args.no_compare:
+        fields_lp = {f: s.get_field(f) for f in s.fields}
+        rec_lp = s.receivers()
+        s32 = make_solver(T, W, m, T.FP32, "0", 0, nt)
+        t32, _ = timed_steps(torch, s32, K, Wn)
+        out["fp32_value"] = m * m * K / t32
+        out["speedup_vs_fp32"] = out["value"] / out["fp32_value"]
+        fields32 = {f: s32.get_field(f) for f in s32.fields}
+        rec32 = s32.receivers()
+        s32.close()
+        s64 = make_solver(T, W, m, T.FP64, "0", 0, nt, model_from=s)
+        s64.step(nt)
+        f64 = {f: s64.get_field(f) for f in s64.fields}
+        rec64 = s64.receivers()
+        s64.close()
+
+        def rl2(a, b):
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        out["rel_l2_vs_fp64"] = {
+            "storage": {"fields": max(rl2(fields_lp[f], f64[f]) for f in f64), "receivers": rl2(rec_lp, rec64)},
+            "fp32": {"fields": max(rl2(fields32[f], f64[f]) for f in f64), "receivers": rl2(rec32, rec64)},
+        }
+    s.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_size or m)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -5,6 +5,8 @@ without FMA contraction, so fields and receiver traces must agree bit for bit
 with the oracle in every storage precision; energies are FP64 sums taken in a
 different order (block tree vs sequential) and agree to 1e-12 relative.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -12,6 +14,23 @@ import common as C
 import twave as T
 
 pytestmark = pytest.mark.gpu
+
+# Kernel variants: streaming kernels with the default chunk of 64 rows, with a
+# 5-row chunk (many chunk boundaries and window warm-ups), and the per-point
+# kernels.  Read by tw_create from the environment.
+KERNELS = [("2", "64"), ("2", "5"), ("1", "64")]
+
+
+@pytest.fixture(params=KERNELS, ids=["stream", "stream-txc5", "point"], autouse=True)
+def kernel_variant(request):
+    old = (os.environ.get("TW_KERNEL"), os.environ.get("TW_TXC"))
+    os.environ["TW_KERNEL"], os.environ["TW_TXC"] = request.param
+    yield request.param
+    for k, v in zip(("TW_KERNEL", "TW_TXC"), old):
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
 
 ENERGY_RTOL = 1e-12
 
@@ -59,10 +78,22 @@ def test_elastic_boundaries(bc, precision):
     run_case("elastic", bc, precision, "6R", mx=36, my=44)
 
 
+@pytest.mark.parametrize("bc,precision", [("FFPP", "fp32"), ("CCCC", "fp64"), ("FCFC", "bf16"), ("PPPP", "fp16"),
+                                          ("PPPP", "fp32"), ("PPFF", "fp64"), ("PPCC", "bf16")])
+def test_elastic_wide(bc, precision):
+    """Grids wider than one 256-column strip: strip halos and ragged last strip."""
+    run_case("elastic", bc, precision, "K" if precision in ("bf16", "fp16") else "6R", mx=21, my=531, nt=12)
+
+
 @pytest.mark.parametrize("bc", ["PPPP", "FFFF", "CCCC", "FCPP", "PPFC"])
 @pytest.mark.parametrize("precision,cpst", [("fp64", "0"), ("fp32", "3R"), ("bf16", "K"), ("fp16", "0")])
 def test_acoustic(bc, precision, cpst):
     run_case("acoustic", bc, precision, cpst, stencil_form=1)
+
+
+@pytest.mark.parametrize("bc", ["PPPP", "CCCC", "PPFC"])
+def test_acoustic_wide(bc):
+    run_case("acoustic", bc, "fp32", "0", mx=17, my=517, nt=12, stencil_form=1)
 
 
 def test_acoustic_reference_energy_form():

@@ -1,0 +1,483 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the parity checker.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign, shift or closure index, a transposed table, a wrong energy
+weight) fails it:
+
+* golden vectors of the reference's own single-core acoustic solver
+  (singlecore/src/main_cpp/cpu_acoustic.cpp, built from its sources by
+  oracle/ref_build.sh, fixtures in tests/golden/), compared at the
+  reference's printed precision;
+* the SBP identity H_N D_MN + (H_M D_NM)^T = e_R p_R^T - e_L p_L^T and the
+  polynomial exactness of the closure tables (namespace_forward.hpp:110-156);
+* skew-adjointness of the assembled one-step operator (interior stencil, SBP
+  closures, free-surface SAT, periodic wrap, material coefficients) in the
+  energy inner product the oracle itself reports, and dissipativity of the
+  characteristic SAT;
+* exact discrete plane waves on periodic grids (per-mode recurrence from the
+  stencil symbol);
+* closed-form energies, the Ricker wavelet's closed-form properties,
+  bilinear-interpolation exactness on linear models, storage rounding against
+  numpy / torch conversions, and compensated vs uncompensated accuracy.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _threads():
+    O.set_threads(min(8, os.cpu_count() or 1))
+
+
+def fmt(v):
+    """The reference prints energies and traces with "% 16.15e"
+    (singlecore/src/main_cpp/cpu_acoustic.cpp:408-428)."""
+    return np.array([float("% 16.15e" % x) for x in v])
+
+
+def reference_acoustic_run(prec, nt):
+    cfg = O.acoustic_reference_config(precision=prec, nt_max=nt)
+    o = O.Oracle(cfg)
+    # source on P at (200, 200), f = 5, t0 = 0.25; receivers at (400, 400)
+    # (singlecore/src/cpp/namespaces.hpp class_mesh::initialize; cpu_acoustic.cpp:88-104)
+    o.set_source_ricker(O.GRID_MM, 200, 200, 5.0, 0.25)
+    o.set_receivers([O.VX, O.VY, O.P], [400, 400, 400], [400, 400, 400])
+    o.step(nt)
+    ek, ep0, ep1 = o.energy()
+    # E = E_Vx + E_Vy + (E_P(n) + E_P(n-1))/2 (cpu_acoustic.cpp:327-344); at it = 0
+    # the previous pressure energy is zero
+    E = (ek + ep1 / 2.0) + ep0 / 2.0
+    return E, o.receivers()
+
+
+@pytest.mark.parametrize("prec,name", [(O.FP64, "f64"), (O.FP32, "f32")])
+def test_golden_energy_head(prec, name):
+    """First 1000 steps of the reference's energy history, digit for digit."""
+    g = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s.npz" % name))
+    n = 1000
+    E, _ = reference_acoustic_run(prec, n)
+    np.testing.assert_array_equal(fmt(E), g["E_head"][:n])
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not os.environ.get("TW_RUN_SLOW"), reason="full 60000-step run (~10 min); set TW_RUN_SLOW=1")
+@pytest.mark.parametrize("prec,name", [(O.FP64, "f64"), (O.FP32, "f32")])
+def test_golden_full_run(prec, name):
+    g = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s.npz" % name))
+    E, rec = reference_acoustic_run(prec, 60000)
+    np.testing.assert_array_equal(fmt(E[::50]), g["E_every"])
+    it = g["trace_it"]
+    for k, f in enumerate(("Vx", "Vy", "P")):
+        np.testing.assert_array_equal(fmt(rec[k][it]), g[f])
+
+
+# ---------------------------------------------------------------------------
+# SBP operators
+# ---------------------------------------------------------------------------
+P_L = np.array([15 / 8, -5 / 4, 3 / 8])  # M-data extrapolation to the boundary node
+
+
+def unit_tables(m):
+    """Oracle's FP64 tables at dx = dt = 1 (so every table is the raw constant)."""
+    cfg = O.make_config(physics=O.ELASTIC, mx=m, my=m, dx=1.0, num_dx=1, den_dx=1, dt=1.0, bc="FFFF",
+                        precision=O.FP64, nt_max=2)
+    o = O.Oracle(cfg)
+    t = {k: o.table(k) for k in range(10)}
+    return t
+
+
+def operators(m):
+    """D_MN: M data -> N points ((m+1) x m), D_NM: N data -> M points (m x (m+1)),
+    assembled from the oracle's closure tables and interior stencil."""
+    t = unit_tables(m)
+    st = t[0]
+    dnl, dnr = t[1].reshape(4, 5), t[2].reshape(4, 5)
+    dml, dmr = t[3].reshape(3, 5), t[4].reshape(3, 5)
+    DMN = np.zeros((m + 1, m))
+    for i in range(m + 1):
+        if i < 4:
+            DMN[i, 0:5] = dnl[i]
+        elif i > m - 4:
+            DMN[i, m - 5:m] = dnr[i - (m - 3)]
+        else:
+            for k, s in enumerate((-2, -1, 0, 1)):
+                DMN[i, i + s] = st[k]
+    DNM = np.zeros((m, m + 1))
+    for i in range(m):
+        if i < 3:
+            DNM[i, 0:5] = dml[i]
+        elif i > m - 4:
+            DNM[i, m + 1 - 5:m + 1] = dmr[i - (m - 3)]
+        else:
+            for k, s in enumerate((-1, 0, 1, 2)):
+                DNM[i, i + s] = st[k]
+    hN = np.ones(m + 1)
+    hN[:4] = t[8]
+    hN[-4:] = t[8][::-1]
+    hM = np.ones(m)
+    hM[:3] = t[9]
+    hM[-3:] = t[9][::-1]
+    return DMN, DNM, np.diag(hN), np.diag(hM)
+
+
+@pytest.mark.parametrize("m", [12, 17])
+def test_sbp_identity(m):
+    DMN, DNM, HN, HM = operators(m)
+    eL = np.zeros(m + 1)
+    eL[0] = 1
+    eR = np.zeros(m + 1)
+    eR[-1] = 1
+    pL = np.zeros(m)
+    pL[:3] = P_L
+    pR = np.zeros(m)
+    pR[-3:] = P_L[::-1]
+    lhs = HN @ DMN + (HM @ DNM).T
+    rhs = np.outer(eR, pR) - np.outer(eL, pL)
+    np.testing.assert_allclose(lhs, rhs, atol=1e-14)
+    # H integrates constants exactly: sum of weights is the length m
+    assert abs(np.trace(HN) - m) < 1e-13 and abs(np.trace(HM) - m) < 1e-13
+
+
+@pytest.mark.parametrize("m", [12, 17])
+def test_polynomial_exactness(m):
+    DMN, DNM, _, _ = operators(m)
+    xn = np.arange(m + 1, dtype=float)
+    xm = np.arange(m) + 0.5
+    for deg in range(4):
+        p = np.poly1d(np.arange(1, deg + 2, dtype=float))
+        dp = p.deriv()
+        # closure rows exact to degree 2, interior to degree 3
+        rows_n = slice(None) if deg <= 2 else slice(4, m - 3)
+        rows_m = slice(None) if deg <= 2 else slice(3, m - 3)
+        np.testing.assert_allclose((DMN @ p(xm))[rows_n], dp(xn)[rows_n], atol=1e-10)
+        np.testing.assert_allclose((DNM @ p(xn))[rows_m], dp(xm)[rows_m], atol=1e-10)
+        if deg <= 2:
+            assert abs(P_L @ p(xm[:3]) - p(0.0)) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# assembled operator: skew-adjointness in the oracle's own energy norm
+# ---------------------------------------------------------------------------
+def random_material(o, physics, rng):
+    ks = [O.C_BINV_VX, O.C_BINV_VY, O.C_MU, O.C_LAMBDA, O.C_TWOMU] if physics == O.ELASTIC else \
+         [O.C_BINV_VX, O.C_BINV_VY, O.C_KAPPA]
+    for k in ks:
+        f = {O.C_BINV_VX: O.VX, O.C_BINV_VY: O.VY, O.C_MU: O.SXY, O.C_LAMBDA: O.SXX, O.C_TWOMU: O.SXX,
+             O.C_KAPPA: O.P}[k]
+        lo, hi = {O.C_LAMBDA: (1.5, 2.5), O.C_TWOMU: (1.6, 2.4)}.get(k, (0.7, 1.3))
+        o.set_coef(k, rng.uniform(lo, hi, o.shape(f)))
+
+
+class OneStep:
+    """Linear one-step map of the oracle (FP64, no source, no compensation)."""
+
+    def __init__(self, physics, bc, m=(9, 8), seed=0):
+        self.physics = physics
+        self.vel = [O.VX, O.VY]
+        self.str = [O.SXX, O.SYY, O.SXY] if physics == O.ELASTIC else [O.P]
+        n = 0
+        self.cfg = O.make_config(physics=physics, mx=m[0], my=m[1], bc=bc, precision=O.FP64, nt_max=4000,
+                                 stencil_form=0)
+        self.o = O.Oracle(self.cfg)
+        random_material(self.o, physics, np.random.default_rng(seed))
+        self.shapes = {f: self.o.shape(f) for f in self.vel + self.str}
+        self.off = {}
+        for f in self.vel + self.str:
+            self.off[f] = n
+            n += int(np.prod(self.shapes[f]))
+        self.n = n
+        self.nv = sum(int(np.prod(self.shapes[f])) for f in self.vel)
+        self.it = 0
+
+    def set(self, x):
+        for f in self.vel + self.str:
+            r, c = self.shapes[f]
+            self.o.set_field(f, x[self.off[f]:self.off[f] + r * c].reshape(r, c))
+
+    def get(self):
+        return np.concatenate([self.o.get_field(f).ravel() for f in self.vel + self.str])
+
+    def step(self, x):
+        self.set(x)
+        self.o.step(1)
+        self.it += 1
+        ek, ep0, ep1 = self.o.energy()
+        return self.get(), ek[self.it - 1], ep0[self.it - 1]
+
+    def matrices(self):
+        """A: stress -> velocity change, B: velocity -> stress change, Dvv:
+        velocity -> velocity change (characteristic SAT), Wv / Ws: energy
+        weights (E = x^T W x / 2)."""
+        nv, n = self.nv, self.n
+        A = np.zeros((nv, n - nv))
+        B = np.zeros((n - nv, nv))
+        Dvv = np.zeros((nv, nv))
+        Wv = np.zeros(nv)
+        Ws = np.zeros((n - nv, n - nv))
+        for k in range(nv):
+            x = np.zeros(n)
+            x[k] = 1.0
+            y, ek, _ = self.step(x)
+            Dvv[:, k] = y[:nv] - x[:nv]
+            B[:, k] = y[nv:]
+            Wv[k] = 2 * ek
+        for j in range(n - nv):
+            x = np.zeros(n)
+            x[nv + j] = 1.0
+            y, _, ep0 = self.step(x)
+            A[:, j] = y[:nv]
+            Ws[j, j] = 2 * ep0
+        if self.physics == O.ELASTIC:  # sxx-syy cross weight at each MM point
+            r, c = self.shapes[O.SXX]
+            for p in range(r * c):
+                x = np.zeros(n)
+                a, b = self.off[O.SXX] + p, self.off[O.SYY] + p
+                x[a] = x[b] = 1.0
+                _, _, ep0 = self.step(x)
+                w = ep0 - Ws[a - nv, a - nv] / 2 - Ws[b - nv, b - nv] / 2
+                Ws[a - nv, b - nv] = Ws[b - nv, a - nv] = w
+        return A, B, Dvv, np.diag(Wv), Ws
+
+
+@pytest.mark.parametrize("physics", [O.ELASTIC, O.ACOUSTIC])
+@pytest.mark.parametrize("bc", ["FFPP", "PPFF", "FFFF", "PPPP"])
+def test_skew_adjoint_conservative(physics, bc):
+    """Free-surface and periodic sides conserve energy: Wv A = -(Ws B)^T."""
+    S = OneStep(physics, bc)
+    A, B, Dvv, Wv, Ws = S.matrices()
+    assert np.abs(Dvv).max() == 0.0
+    lhs = Wv @ A
+    scale = np.abs(lhs).max()
+    assert scale > 0
+    np.testing.assert_allclose(lhs, -(Ws @ B).T, atol=1e-13 * scale)
+    # weights are positive on every point the energy counts
+    assert np.all(np.linalg.eigvalsh(Ws) > -1e-15) and np.all(np.diag(Wv) >= 0)
+
+
+@pytest.mark.parametrize("physics", [O.ELASTIC, O.ACOUSTIC])
+@pytest.mark.parametrize("bc", ["CCCC", "FCPP", "PPCF"])
+def test_characteristic_dissipative(physics, bc):
+    """Characteristic sides keep the stress->velocity coupling skew and add a
+    velocity self-term whose symmetric part (in the energy norm) is negative
+    semidefinite and nonzero."""
+    S = OneStep(physics, bc, seed=3)
+    A, B, Dvv, _, _ = S.matrices()
+    # the characteristic SAT changes a velocity-only start within the step, so
+    # take the energy weights (the same H-norm) from the free-surface twin
+    _, _, _, Wv, Ws = OneStep(physics, bc.replace("C", "F"), seed=3).matrices()
+    # with a velocity-only start, B sees the velocity after the dissipative SAT
+    B = B @ np.linalg.inv(np.eye(S.nv) + Dvv)
+    lhs = Wv @ A
+    np.testing.assert_allclose(lhs, -(Ws @ B).T, atol=1e-12 * np.abs(lhs).max())
+    sym = Wv @ Dvv + (Wv @ Dvv).T
+    ev = np.linalg.eigvalsh(sym)
+    assert ev.max() < 1e-14 * np.abs(ev).max()
+    assert ev.min() < 0
+
+
+# ---------------------------------------------------------------------------
+# exact discrete plane waves on periodic grids
+# ---------------------------------------------------------------------------
+def symbol(kappa):
+    """i-free part of the staggered derivative symbol: D e^{ikx} = 2i s(k) e^{ikx}
+    (stencil 1/24, -9/8, 9/8, -1/24)."""
+    return (9 / 8) * np.sin(kappa / 2) - (1 / 24) * np.sin(3 * kappa / 2)
+
+
+def positions(field, shape):
+    """Grid coordinates (in cells) of a field: N-type axes at integers, M-type
+    at half integers (VX on NM, VY on MN, SXX/SYY/P on MM, SXY on NN)."""
+    tx = {O.VX: 1, O.VY: 0, O.SXX: 0, O.SYY: 0, O.SXY: 1, O.P: 0}[field]
+    ty = {O.VX: 0, O.VY: 1, O.SXX: 0, O.SYY: 0, O.SXY: 1, O.P: 0}[field]
+    x = np.arange(shape[0]) + (0 if tx else 0.5)
+    y = np.arange(shape[1]) + (0 if ty else 0.5)
+    return np.meshgrid(x, y, indexing="ij")
+
+
+@pytest.mark.parametrize("physics", [O.ELASTIC, O.ACOUSTIC])
+def test_plane_wave(physics):
+    m, kx, ky, nt = 16, 2, 3, 40
+    dx = 1.0 / 64
+    dt = 0.4 * dx / (np.sqrt(2) * 2.0)
+    cfg = O.make_config(physics=physics, mx=m, my=m, dx=dx, num_dx=1, den_dx=64, dt=dt, bc="PPPP",
+                        precision=O.FP64, nt_max=nt, stencil_form=0)
+    o = O.Oracle(cfg)
+    b, lam, mu, kap = 0.8, 1.7, 0.9, 2.3
+    fields = [O.VX, O.VY, O.SXX, O.SYY, O.SXY] if physics == O.ELASTIC else [O.VX, O.VY, O.P]
+    for k, v in ((O.C_BINV_VX, b), (O.C_BINV_VY, b)):
+        o.set_coef(k, np.full(o.shape(O.VX if k == O.C_BINV_VX else O.VY), v))
+    if physics == O.ELASTIC:
+        o.set_coef(O.C_MU, np.full(o.shape(O.SXY), mu))
+        o.set_coef(O.C_LAMBDA, np.full(o.shape(O.SXX), lam))
+        o.set_coef(O.C_TWOMU, np.full(o.shape(O.SXX), 2 * mu))
+    else:
+        o.set_coef(O.C_KAPPA, np.full(o.shape(O.P), kap))
+    ax, ay = 2 * np.pi * kx / m, 2 * np.pi * ky / m
+    rng = np.random.default_rng(7)
+    amp = {f: complex(*rng.standard_normal(2)) for f in fields}
+    phase = {}
+    for f in fields:
+        X, Y = positions(f, o.shape(f))
+        phase[f] = np.exp(1j * (ax * X + ay * Y))
+        o.set_field(f, (amp[f] * phase[f]).real)
+    o.step(nt)
+    r = dt / dx
+    sx, sy = 2j * symbol(ax), 2j * symbol(ay)
+    a = dict(amp)
+    for _ in range(nt):
+        if physics == O.ELASTIC:
+            a[O.VX] += r * b * (sx * a[O.SXX] + sy * a[O.SXY])
+            a[O.VY] += r * b * (sx * a[O.SXY] + sy * a[O.SYY])
+            a[O.SXX] += r * ((lam + 2 * mu) * sx * a[O.VX] + lam * sy * a[O.VY])
+            a[O.SYY] += r * (lam * sx * a[O.VX] + (lam + 2 * mu) * sy * a[O.VY])
+            a[O.SXY] += r * mu * (sx * a[O.VY] + sy * a[O.VX])
+        else:
+            a[O.VX] += r * b * sx * a[O.P]
+            a[O.VY] += r * b * sy * a[O.P]
+            a[O.P] += r * kap * (sx * a[O.VX] + sy * a[O.VY])
+    for f in fields:
+        np.testing.assert_allclose(o.get_field(f), (a[f] * phase[f]).real, atol=1e-12, err_msg=str(f))
+
+
+# ---------------------------------------------------------------------------
+# energy, source, interpolation, storage, compensation
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("bc", ["FFFF", "FFPP", "PPPP"])
+def test_energy_closed_form(bc):
+    """Uniform fields in a uniform medium: the H-norm integrates constants
+    exactly, so E_k = rho v^2 L^2 / 2 and E_p = (sigma C^-1 sigma) L^2 / 2."""
+    m, dx = 20, 0.05
+    L = m * dx
+    cfg = O.make_config(physics=O.ELASTIC, mx=m, my=m, dx=dx, num_dx=1, den_dx=20, dt=1e-3, bc=bc,
+                        precision=O.FP64, nt_max=2)
+    o = O.Oracle(cfg)
+    rho, lam, mu = 1.6, 2.2, 0.7
+    o.set_coef(O.C_BINV_VX, np.full(o.shape(O.VX), 1 / rho))
+    o.set_coef(O.C_BINV_VY, np.full(o.shape(O.VY), 1 / rho))
+    o.set_coef(O.C_MU, np.full(o.shape(O.SXY), mu))
+    o.set_coef(O.C_LAMBDA, np.full(o.shape(O.SXX), lam))
+    o.set_coef(O.C_TWOMU, np.full(o.shape(O.SXX), 2 * mu))
+    vx, vy, sxx, syy, sxy = 0.3, -0.2, 0.5, 0.25, 0.4
+    for f, v in ((O.VX, vx), (O.VY, vy), (O.SXX, sxx), (O.SYY, syy), (O.SXY, sxy)):
+        o.set_field(f, np.full(o.shape(f), v))
+    o.step(1)
+    ek, ep0, _ = o.energy()
+    # constant fields have zero derivative in the interior and periodic
+    # directions; velocities change only through free-surface SAT terms, so
+    # compare E_k only where no free surface acts on them
+    D = 4 * mu * (lam + mu)
+    compl = np.array([[lam + 2 * mu, -lam], [-lam, lam + 2 * mu]]) / D
+    s = np.array([sxx, syy])
+    ep_exact = 0.5 * L * L * (s @ compl @ s + sxy * sxy / mu)
+    np.testing.assert_allclose(ep0[0], ep_exact, rtol=1e-13)
+    if bc == "PPPP":
+        np.testing.assert_allclose(ek[0], 0.5 * L * L * rho * (vx * vx + vy * vy), rtol=1e-13)
+
+
+def test_ricker_properties():
+    """A = (1 - 2 pi^2 f^2 t^2) exp(-pi^2 f^2 t^2): peak 1 at t = t0, zeros at
+    t0 +- 1/(pi f sqrt 2), minimum -2 exp(-3/2) at t0 +- sqrt(3/2)/(pi f),
+    zero mean; increment = A dt / dx^2 / source_scaling."""
+    dx, dt, f0 = 0.01, 1e-4, 10.0
+    t0 = 0.2
+    cfg = O.make_config(dx=dx, dt=dt, source_scaling=2.0)
+    inc = lambda it, half=0: O.ricker_increment(cfg, f0, t0, it, half)
+    k = dt / dx ** 2 / 2.0
+    assert abs(inc(2000) / k - 1.0) < 1e-15
+    # half step: t = (it + 1/2) dt - t0 = -dt/2, where A = 1 - 3 pi^2 f^2 t^2 + O(t^4)
+    th = dt / 2
+    assert inc(1999, 1) / k == pytest.approx(1 - 3 * (np.pi * f0 * th) ** 2, abs=1e-10)
+    tz = 1 / (np.pi * f0 * np.sqrt(2))
+    # sign change across the zero crossing
+    iz = int(np.floor((t0 + tz) / dt))
+    assert inc(iz) > 0 > inc(iz + 1)
+    tmin = np.sqrt(1.5) / (np.pi * f0)
+    imin = int(round((t0 + tmin) / dt))
+    assert inc(imin) / k == pytest.approx(-2 * np.exp(-1.5), rel=1e-6)
+    total = sum(inc(i) for i in range(0, 4001)) / k
+    assert abs(total) < 1e-9 * 4000
+
+
+@pytest.mark.parametrize("grid", [O.GRID_NN, O.GRID_MN, O.GRID_NM, O.GRID_MM])
+def test_interpolation_linear_exact(grid):
+    """Bilinear interpolation reproduces a linear model exactly at every
+    staggered point (grid_parameter.cpp:6-60); model spacing 2 dx."""
+    mx, my = 12, 16
+    cfg = O.make_config(mx=mx, my=my, dx=1 / 100, num_dx=1, den_dx=100)
+    nxm, nym = mx // 2, my // 2
+    X, Y = np.meshgrid(np.arange(nxm + 1) * 2.0, np.arange(nym + 1) * 2.0, indexing="ij")
+    model = 3.0 + 0.25 * X - 0.5 * Y
+    out = O.interpolate(cfg, grid, model, 1, 50)
+    tx, ty = grid in (O.GRID_NN, O.GRID_NM), grid in (O.GRID_NN, O.GRID_MN)
+    x = np.arange(out.shape[0]) + (0 if tx else 0.5)
+    y = np.arange(out.shape[1]) + (0 if ty else 0.5)
+    Xg, Yg = np.meshgrid(x, y, indexing="ij")
+    np.testing.assert_allclose(out, 3.0 + 0.25 * Xg - 0.5 * Yg, atol=1e-12)
+
+
+def test_storage_rounding():
+    """State rounding of the 16/32-bit oracles equals the library conversions
+    (round to nearest even), including ties and subnormals."""
+    import torch
+    rng = np.random.default_rng(11)
+    v = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-6, 4, 2000),
+                        # bf16 ties: 1 + k 2^-8 + 2^-9 in float32
+                        1.0 + np.arange(64) * 2.0 ** -7 + 2.0 ** -8,
+                        np.array([0.0, -0.0, 6.0e-8, 1e-40, 65504.0, 3.0e38])])
+    m = 1 + int(np.sqrt(v.size))
+    vals = np.zeros((m, m))
+    vals.ravel()[:v.size] = v
+    cases = [(O.FP32, lambda a: a.astype(np.float32).astype(np.float64)),
+             (O.FP16, lambda a: a.astype(np.float32).astype(np.float16).astype(np.float64)),
+             (O.BF16, lambda a: torch.from_numpy(a.astype(np.float32)).to(torch.bfloat16).double().numpy())]
+    for prec, conv in cases:
+        cfg = O.make_config(physics=O.ELASTIC, mx=m, my=m - 1, precision=prec, nt_max=2)
+        o = O.Oracle(cfg)
+        o.set_field(O.VX, vals[:m + 1, :m - 1] if vals.shape[0] > m else np.pad(vals, ((0, 1), (0, 0)))[:, :m - 1])
+        got = o.get_field(O.VX)
+        want = conv(np.pad(vals, ((0, 1), (0, 0)))[:, :m - 1])
+        with np.errstate(over="ignore"):
+            np.testing.assert_array_equal(got, want, err_msg="precision %d" % prec)
+
+
+def compensation_error(prec, cpst, physics):
+    """Relative L2 error vs FP64 after 400 small steps (CFL 0.05) of a smooth
+    pulse: the regime where the rounding of the accumulation dominates."""
+    m, nt = 48, 400
+    dx = 1 / 125
+    dt = 0.05 * dx / (np.sqrt(2) * 2)
+
+    def run(p, c):
+        cfg = O.make_config(physics=physics, mx=m, my=m, dx=dx, dt=dt, bc="FFPP", precision=p, cpst=c, nt_max=nt,
+                            stencil_form=physics)
+        o = O.Oracle(cfg)
+        fs = [O.VX, O.VY, O.SXX, O.SYY, O.SXY] if physics == O.ELASTIC else [O.VX, O.VY, O.P]
+        for f in fs:
+            r, cc = o.shape(f)
+            X, Y = np.meshgrid(np.arange(r) / m, np.arange(cc) / m, indexing="ij")
+            amp = 1.0 if f in (O.SXX, O.SYY, O.P) else 0.0
+            o.set_field(f, amp * np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2) / 0.01))
+        o.step(nt)
+        return {f: o.get_field(f) for f in fs}
+
+    ref = run(O.FP64, "0")
+    got = run(prec, cpst)
+    return max(np.linalg.norm(got[f] - ref[f]) / np.linalg.norm(ref[f]) for f in ref)
+
+
+@pytest.mark.parametrize("physics", [O.ELASTIC, O.ACOUSTIC])
+def test_compensation_reduces_error(physics):
+    e0 = compensation_error(O.FP32, "0", physics)
+    assert 1e-7 < e0 < 1e-5
+    for c in ("3R", "-3R", "6R", "3C", "-3C", "6C"):
+        assert compensation_error(O.FP32, c, physics) < 0.3 * e0, c
+    for prec, ratio in ((O.BF16, 0.1), (O.FP16, 0.2)):
+        e16 = compensation_error(prec, "0", physics)
+        assert compensation_error(prec, "K", physics) < ratio * e16

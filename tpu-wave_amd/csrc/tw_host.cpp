@@ -10,6 +10,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <stdlib.h>
+
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -504,6 +507,8 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   memcpy(p.bc, c->bc, 4);
   p.energy_form = c->energy_form;
   p.dx = c->dx;
+  if (const char* kv = getenv("TW_KERNEL")) p.kver = atoi(kv) == 1 ? 1 : 2;
+  if (const char* tx = getenv("TW_TXC")) p.txc = atoi(tx) > 0 ? atoi(tx) : 64;
   build_tables(c, &p.t);
   const size_t es = state_size(c->precision);
   auto bail = [&](int code) {
@@ -552,11 +557,21 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   if ((e = dalloc(s, (void**)&p.ek, nt * 8)) || (e = dalloc(s, (void**)&p.ep0, nt * 8)) ||
       (e = dalloc(s, (void**)&p.ep1, nt * 8)))
     return bail(e);
-  dim3 g, b;
-  tw::grid_dims(p, &g, &b);
-  p.nparts = (int32_t)(g.x * g.y);
-  if ((e = dalloc(s, (void**)&p.part_v, (size_t)p.nparts * 8)) || (e = dalloc(s, (void**)&p.part_s, (size_t)p.nparts * 8)))
+  p.nparts = tw::half_parts(p);
+  const size_t npart_max = (size_t)std::max(p.nparts, tw::energy_parts(p));
+  if ((e = dalloc(s, (void**)&p.part_v, npart_max * 8)) || (e = dalloc(s, (void**)&p.part_s, npart_max * 8)))
     return bail(e);
+  {
+    const size_t sb = tw::stat_fill(p, nullptr);
+    std::vector<uint8_t> hb(sb);
+    tw::stat_fill(p, hb.data());
+    if ((e = dalloc(s, &p.dstat, sb))) return bail(e);
+    he = hipMemcpy(p.dstat, hb.data(), sb, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+      hip_fail(s, he, "hipMemcpy");
+      return bail(TW_EDEVICE);
+    }
+  }
   he = hipStreamCreate(&s->stream);
   if (he != hipSuccess) {
     hip_fail(s, he, "hipStreamCreate");
@@ -744,7 +759,7 @@ int tw_step(tw_solver* s, int32_t nsteps) {
       HIPCHK(s, tw::launch_stress_energy(p, s->stream));
       carry = tw::Pending();
       carry.part = p.part_s;
-      carry.nparts = p.nparts;
+      carry.nparts = tw::energy_parts(p);
       carry.out0 = p.ep0;
       carry.it0 = it;
       carry.scale = c->energy_form == 1 ? 1.0 : 0.5;

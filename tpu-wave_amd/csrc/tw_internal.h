@@ -66,6 +66,9 @@ struct Plan {
   int32_t perx, pery;
   char bc[4];
   int32_t energy_form;
+  int32_t kver = 2;   // 1: per-point kernels, 2: streaming kernels
+  void* dstat = nullptr;  // device copy of the static kernel arguments
+  int32_t txc = 64;   // rows per chunk of the streaming kernels
   double dx;
   Tables t;
   DevField f[NF];
@@ -101,6 +104,10 @@ struct StepCtl {
 
 // launches (tw_kernels.hip)
 void grid_dims(const Plan& p, dim3* grid, dim3* block);
+int energy_parts(const Plan& p);  // partial sums written by launch_stress_energy
+int half_parts(const Plan& p);    // partial sums written per half step
+// bytes of the static kernel arguments; fills `out` (host memory) when non-null
+size_t stat_fill(const Plan& p, void* out);
 hipError_t launch_half(const Plan& p, int half, const StepCtl& c, hipStream_t s);
 hipError_t launch_uhat(const Plan& p, hipStream_t s);
 hipError_t launch_stress_energy(const Plan& p, hipStream_t s);

@@ -21,6 +21,9 @@
 // result deterministic without atomics or an extra launch.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+
+#include <type_traits>
 
 #include "tw_internal.h"
 
@@ -50,15 +53,11 @@ struct St<bf16> {
   using C = float;
   static __device__ __forceinline__ float ld(bf16 v) { return __uint_as_float((uint32_t)v.u << 16); }
   // round to nearest even, NaN kept quiet (same rule as the host conversion)
+  // round to nearest even (gfx950 v_cvt_pk_bf16_f32; FP32 denormals are kept,
+  // so every finite value rounds as the host's bit-level RNE does)
   static __device__ __forceinline__ bf16 st(float f) {
-    uint32_t u = __float_as_uint(f);
     bf16 r;
-    if ((u & 0x7fffffffu) > 0x7f800000u) {
-      r.u = (uint16_t)((u >> 16) | 0x40u);
-      return r;
-    }
-    u += 0x7fffu + ((u >> 16) & 1u);
-    r.u = (uint16_t)(u >> 16);
+    r.u = __builtin_bit_cast(uint16_t, (__bf16)f);
     return r;
   }
 };
@@ -101,8 +100,11 @@ struct K {
   __device__ __forceinline__ KT get(int ix, int iy) const { return p[(int64_t)ix * pitch + iy]; }
 };
 
+// Solver description that does not change between launches.  The streaming
+// kernels read it from a device copy (so out-of-line helpers can take its
+// address); the per-point kernels receive it by value inside Args.
 template <class ST, class KT, class CT>
-struct Args {
+struct Stat {
   F<ST> f[NF];
   K<KT> k[NF];
   TabT<CT> t;
@@ -113,7 +115,10 @@ struct Args {
   int32_t cpst, eform, physics;
   const CT* z[2][4];
   CT* uhat[2][4];
-  // step control
+};
+
+// Per-launch control.
+struct Dyn {
   int32_t it, energy;
   int32_t src_on, src_grid, src_ix, src_iy;
   double src_inc;
@@ -122,6 +127,9 @@ struct Args {
   double* rec;
   int32_t nt_max;
 };
+
+template <class ST, class KT, class CT>
+struct Args : Stat<ST, KT, CT>, Dyn {};
 
 template <class ST, class KT, class CT>
 static Args<ST, KT, CT> make_args(const Plan& p, const StepCtl* c, double* part) {
@@ -268,7 +276,7 @@ __device__ __forceinline__ CT ddy(const F<ST>& P, int out_t, int ix, int iy, int
 // extrapolation for an M-type output.  Returns the increment to add to acc,
 // already multiplied by the coefficient, or leaves acc unchanged.
 template <class ST, class KT, class CT>
-__device__ __forceinline__ CT sat_add(const Args<ST, KT, CT>& A, CT acc, int vf, int pf, int axis, int out_t, int ix,
+__device__ __forceinline__ CT sat_add(const Stat<ST, KT, CT>& A, CT acc, int vf, int pf, int axis, int out_t, int ix,
                                       int iy, CT own, CT coef) {
   using S = St<ST>;
   const F<ST>& P = A.f[pf];
@@ -327,7 +335,7 @@ template <class ST, class CT>
 __device__ __forceinline__ void update(const F<ST>& X, int64_t q, CT acc, int cpst, ST* snew, ST* rnew) {
   using S = St<ST>;
   if constexpr (sizeof(ST) >= 4) {
-    const CT s = S::ld(X.s[q]);
+    const CT s = S::ld(*snew);  // old state (caller loads it)
     const CT r = acc;
     switch (cpst) {
       case TW_CPST_0:
@@ -383,7 +391,7 @@ __device__ __forceinline__ void update(const F<ST>& X, int64_t q, CT acc, int cp
       } break;
     }
   } else {
-    const CT t = S::ld(X.s[q]) + acc;
+    const CT t = S::ld(*snew) + acc;
     const ST hi = S::st(t);
     *snew = hi;
     if (cpst == TW_CPST_K) *rnew = S::st(t - S::ld(hi));
@@ -421,7 +429,7 @@ __device__ __forceinline__ CT value_of(ST s, ST r, int kmode) {
 // H-norm factor along an axis (adjust_parameters_energy_periodic,
 // cuda/src/cpp/grid_energy.cpp:58-177)
 template <class ST, class KT, class CT>
-__device__ __forceinline__ double hw(const Args<ST, KT, CT>& A, int axis, int t, int i, int n) {
+__device__ __forceinline__ double hw(const Stat<ST, KT, CT>& A, int axis, int t, int i, int n) {
   const int periodic = axis == 0 ? A.perx : A.pery;
   if (periodic) return (t && i == n - 1) ? 0.0 : A.dx;
   const int nb = t ? 4 : 3;
@@ -432,7 +440,7 @@ __device__ __forceinline__ double hw(const Args<ST, KT, CT>& A, int axis, int t,
 
 // energy term of one scalar field point (grid_energy.cpp:180-216 / cpu_acoustic.cpp:327-344)
 template <class ST, class KT, class CT>
-__device__ __forceinline__ double energy_pt(const Args<ST, KT, CT>& A, CT v, CT prm, int tx, int ty, int ix, int iy,
+__device__ __forceinline__ double energy_pt(const Stat<ST, KT, CT>& A, CT v, CT prm, int tx, int ty, int ix, int iy,
                                             int rows, int cols) {
   if (A.eform == 1) {
     double w = A.dx * A.dx / 2.;
@@ -471,8 +479,8 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 
 // work left by the previous launch, done by block 0
 template <class ST, class KT, class CT>
-__device__ void do_pending(const Args<ST, KT, CT>& A, double* sh) {
-  const Pending& P = A.pend;
+__device__ void do_pending(const Stat<ST, KT, CT>& A, const Dyn& D, double* sh) {
+  const Pending& P = D.pend;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int nth = blockDim.x * blockDim.y;
   if (P.nparts > 0) {
@@ -492,25 +500,54 @@ __device__ void do_pending(const Args<ST, KT, CT>& A, double* sh) {
     const int64_t q = X.at(rc.ix, rc.iy);
     ST rr = X.s[q];
     if (kmode) rr = X.r[q];
-    A.rec[(int64_t)rc.row * A.nt_max + P.rcv_it] = (double)value_of<ST, CT>(X.s[q], rr, kmode);
+    D.rec[(int64_t)rc.row * D.nt_max + P.rcv_it] = (double)value_of<ST, CT>(X.s[q], rr, kmode);
   }
 }
 
 // ------------------------------------------------------------ half steps
+// Rectangles of the union index space covered by a per-point launch.  n = 0:
+// the whole (mx+1) x (my+1) space on a 2-D grid.  Otherwise blockIdx.x runs
+// over the 64x4 tiles of the n rectangles in order (the SBP boundary band
+// around the core region of the streaming kernels).
+struct Rects {
+  int32_t n;
+  int32_t r0[4], r1[4], c0[4], c1[4];
+  int32_t cum[5];  // first tile of each rectangle
+  int32_t poff;    // offset of this launch's energy partials
+};
+
+__device__ __forceinline__ bool map_point(const Rects& R, int* ix, int* iy, int* pidx) {
+  if (R.n == 0) {
+    *iy = blockIdx.x * blockDim.x + threadIdx.x;
+    *ix = blockIdx.y * blockDim.y + threadIdx.y;
+    *pidx = blockIdx.y * gridDim.x + blockIdx.x;
+    return true;
+  }
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < R.n && b >= R.cum[k + 1]) k++;
+  const int t = b - R.cum[k];
+  const int tc = (R.c1[k] - R.c0[k] + 63) / 64;
+  *ix = R.r0[k] + (t / tc) * 4 + threadIdx.y;
+  *iy = R.c0[k] + (t % tc) * 64 + threadIdx.x;
+  *pidx = R.poff + b;
+  return *ix < R.r1[k] && *iy < R.c1[k];
+}
+
 // Velocity half step: vx (NM) and vy (MN).
 template <class ST, class KT, class CT>
-__global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A) {
+__global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
   __shared__ double sh[8];
-  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending(A, sh);
-  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ix = blockIdx.y * blockDim.y + threadIdx.y;
+  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, A, sh);
+  int ix, iy, pidx;
+  const bool inr = map_point(R, &ix, &iy, &pidx);
   const int el = A.physics == TW_ELASTIC;
   const int kmode = A.cpst == TW_CPST_K;
   double e = 0.0;
   // ---- vx on NM: d/dx of sxx (elastic) or p, + d/dy of sxy (elastic)
   {
     const F<ST>& X = A.f[TW_VX];
-    if (ix < X.rows && iy < X.cols) {
+    if (inr && ix < X.rows && iy < X.cols) {
       const int64_t q = X.at(ix, iy);
       const CT b = kld<KT, CT>(A.k[TW_C_BINV_VX].get(ix, iy));
       const CT own = (CT)St<ST>::ld(X.s[q]);
@@ -537,7 +574,7 @@ __global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A) {
   // ---- vy on MN: d/dx of sxy (elastic) + d/dy of syy (elastic) or p
   {
     const F<ST>& X = A.f[TW_VY];
-    if (ix < X.rows && iy < X.cols) {
+    if (inr && ix < X.rows && iy < X.cols) {
       const int64_t q = X.at(ix, iy);
       const CT b = kld<KT, CT>(A.k[TW_C_BINV_VY].get(ix, iy));
       const CT own = (CT)St<ST>::ld(X.s[q]);
@@ -563,24 +600,24 @@ __global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A) {
   }
   if (A.energy) {
     const double bsum = block_sum(e, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) A.part[blockIdx.y * gridDim.x + blockIdx.x] = bsum;
+    if (threadIdx.x == 0 && threadIdx.y == 0) A.part[pidx] = bsum;
   }
 }
 
 // Stress half step: sxy (NN) and sxx, syy (MM), or p (MM, acoustic).
 template <class ST, class KT, class CT>
-__global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A) {
+__global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
   __shared__ double sh[8];
-  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending(A, sh);
-  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ix = blockIdx.y * blockDim.y + threadIdx.y;
+  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, A, sh);
+  int ix, iy, pidx;
+  const bool inr = map_point(R, &ix, &iy, &pidx);
   const int kmode = A.cpst == TW_CPST_K;
   const int rmode = kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R;
   double e = 0.0;
   if (A.physics == TW_ELASTIC) {
     {
       const F<ST>& X = A.f[TW_SXY];
-      if (ix < X.rows && iy < X.cols) {
+      if (inr && ix < X.rows && iy < X.cols) {
         const int64_t q = X.at(ix, iy);
         const CT mu = kld<KT, CT>(A.k[TW_C_MU].get(ix, iy));
         CT acc = acc_init<ST, CT>(X, q, A.cpst);
@@ -601,7 +638,7 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A) {
     {
       const F<ST>& X0 = A.f[TW_SXX];
       const F<ST>& X1 = A.f[TW_SYY];
-      if (ix < X0.rows && iy < X0.cols) {
+      if (inr && ix < X0.rows && iy < X0.cols) {
         const int64_t q = X0.at(ix, iy);
         const CT lam = kld<KT, CT>(A.k[TW_C_LAMBDA].get(ix, iy));
         const CT tmu = kld<KT, CT>(A.k[TW_C_TWOMU].get(ix, iy));
@@ -647,7 +684,7 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A) {
     }
   } else {
     const F<ST>& X = A.f[TW_P];
-    if (ix < X.rows && iy < X.cols) {
+    if (inr && ix < X.rows && iy < X.cols) {
       const int64_t q = X.at(ix, iy);
       const CT kap = kld<KT, CT>(A.k[TW_C_KAPPA].get(ix, iy));
       CT acc = acc_init<ST, CT>(X, q, A.cpst);
@@ -671,7 +708,7 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A) {
   }
   if (A.energy) {
     const double bsum = block_sum(e, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) A.part[blockIdx.y * gridDim.x + blockIdx.x] = bsum;
+    if (threadIdx.x == 0 && threadIdx.y == 0) A.part[pidx] = bsum;
   }
 }
 
@@ -755,7 +792,7 @@ __global__ void __launch_bounds__(256) k_uhat(Args<ST, KT, CT> A) {
 template <class ST, class KT, class CT>
 __global__ void __launch_bounds__(256) k_finalize(Args<ST, KT, CT> A) {
   __shared__ double sh[8];
-  do_pending(A, sh);
+  do_pending<ST, KT, CT>(A, A, sh);
 }
 
 template <class ST>
@@ -768,28 +805,902 @@ __global__ void __launch_bounds__(256) k_finite(F<ST> X, int32_t* flag) {
   }
 }
 
+// ============================================================ streaming kernels
+// One wavefront (a 64-thread block) owns a strip of 256 columns (4 per lane)
+// and walks down a chunk of rows.  Partner rows needed by the x stencil stay
+// in a rolling register window, so each is read from HBM once per chunk; the
+// y stencil reads a per-row LDS segment of the y partner (strip + 8-column
+// halo each side, periodic wrap resolved when the segment is filled).  SBP
+// closure rows read their five partner rows directly; closure columns use the
+// five-tap boundary rows on the LDS segment.  The per-point arithmetic is the
+// same sequence as the per-point kernels above.
+constexpr int VW = 4;
+constexpr int STRIP = 64 * VW;
+constexpr int HALO = 8;
+constexpr int SEG = STRIP + 2 * HALO;
+
+template <class ST>
+struct alignas(4 * sizeof(ST)) V4 {
+  ST v[4];
+};
+
+// raw integer vectors of the same size, for loads/stores through typed
+// address-space pointers (struct copies cannot go through them)
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <int B>
+struct RawOf;
+template <>
+struct RawOf<2> {
+  using e = uint16_t;
+  using v = u32x2;
+};
+template <>
+struct RawOf<4> {
+  using e = uint32_t;
+  using v = u32x4;
+};
+template <>
+struct RawOf<8> {
+  using e = uint64_t;
+  struct alignas(32) v {
+    u32x4 a, b;
+  };
+};
+
+template <class ST, class PT>
+__device__ __forceinline__ V4<ST> ld_v4(PT p) {
+  V4<ST> o;
+  if constexpr (sizeof(ST) == 8) {
+    typedef __attribute__((address_space(1))) const u32x4* Q;
+    const u32x4 a = ((Q)p)[0], b = ((Q)p)[1];
+    __builtin_memcpy(&o.v[0], &a, 16);
+    __builtin_memcpy(&o.v[2], &b, 16);
+  } else {
+    typedef typename RawOf<sizeof(ST)>::v R;
+    typedef __attribute__((address_space(1))) const R* Q;
+    const R r = *(Q)p;
+    __builtin_memcpy(&o, &r, sizeof o);
+  }
+  return o;
+}
+
+template <class ST, class PT>
+__device__ __forceinline__ void st_v4(PT p, const V4<ST>& v) {
+  if constexpr (sizeof(ST) == 8) {
+    typedef __attribute__((address_space(1))) u32x4* Q;
+    u32x4 a, b;
+    __builtin_memcpy(&a, &v.v[0], 16);
+    __builtin_memcpy(&b, &v.v[2], 16);
+    ((Q)p)[0] = a;
+    ((Q)p)[1] = b;
+  } else {
+    typedef typename RawOf<sizeof(ST)>::v R;
+    typedef __attribute__((address_space(1))) R* Q;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof r);
+    *(Q)p = r;
+  }
+}
+
+template <class ST, class PT>
+__device__ __forceinline__ void st_1(PT p, const ST& v) {
+  typedef typename RawOf<sizeof(ST)>::e E;
+  typedef __attribute__((address_space(1))) E* Q;
+  E e;
+  __builtin_memcpy(&e, &v, sizeof e);
+  *(Q)p = e;
+}
+
+// columns c..c+3 of a row (c multiple of 4, row valid) -> arithmetic type
+template <class ST, class CT, class PT>
+__device__ __forceinline__ void ld4(PT rowp, int c, CT o[4]) {
+  const V4<ST> t = ld_v4<ST>(rowp + c);
+#pragma unroll
+  for (int j = 0; j < 4; j++) o[j] = (CT)St<ST>::ld(t.v[j]);
+}
+
+
+// value of partner column v of a row, with periodic mapping / bounds
+template <class ST, class CT>
+__device__ __forceinline__ CT ldcol(const ST* rowp, int v, int periodic, int M, int ncols) {
+  if (periodic) v = ((v % M) + M) % M;
+  if (v < 0 || v >= ncols) return (CT)0;
+  return (CT)St<ST>::ld(rowp[v]);
+}
+
+// fill the LDS segment of partner row `row` for virtual columns base-HALO .. base+STRIP+HALO
+template <class ST, class CT>
+__device__ __forceinline__ void seg_fill(CT* seg, const F<ST>& P, int row, int base, int lane, int periodic, int M) {
+  const ST* rowp = P.s + (int64_t)row * P.pitch;
+  const int v0 = base + lane * VW;
+  CT o[4];
+  if (v0 >= 0 && v0 + 3 < (periodic ? M : P.cols)) {
+    ld4<ST, CT>(rowp, v0, o);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) o[j] = ldcol<ST, CT>(rowp, v0 + j, periodic, M, P.cols);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) seg[HALO + lane * VW + j] = o[j];
+  if (lane < 2) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) seg[lane * 4 + j] = ldcol<ST, CT>(rowp, base - HALO + lane * 4 + j, periodic, M, P.cols);
+  } else if (lane >= 62) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      seg[HALO + STRIP + (lane - 62) * 4 + j] = ldcol<ST, CT>(rowp, base + STRIP + (lane - 62) * 4 + j, periodic, M, P.cols);
+  }
+}
+
+// interior x derivative from the register window (rows in tap order)
+template <class CT>
+__device__ __forceinline__ CT dx_win(const CT w[4][4], int j, const CT st[4]) {
+  CT v = (CT)0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) v = v + w[k][j] * st[k];
+  return v;
+}
+
+// interior / periodic y derivative from the LDS segment
+template <class CT>
+__device__ __forceinline__ CT dy_win(const CT* seg, int base, int col, int out_t, const CT st[4]) {
+  CT v = (CT)0;
+  const int p = col - base + HALO + (out_t ? -2 : -1);
+#pragma unroll
+  for (int k = 0; k < 4; k++) v = v + seg[p + k] * st[k];
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ void shift_win(T w[4][4]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) w[k][j] = w[k + 1][j];
+}
+
+// Core region of a half step: rows [rlo, rhi) and columns [clo, chi) of the
+// union index space, away from SBP closures (all rows / columns along a
+// periodic axis).  The boundary band around it is done by the per-point
+// kernels launched over the band rectangles.
+struct Core {
+  int32_t rlo, rhi, clo, chi, txc;
+};
+
+// field parameters held in registers for the whole kernel
+// global (address space 1) pointer: the field pointers come from a struct in
+// memory, so without the cast the compiler would emit flat accesses
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
+}
+
+template <class ST>
+struct FR {
+  gptr<ST> s;
+  gptr<ST> r;
+  int64_t pitch;
+  int32_t rows, cols;
+  __device__ __forceinline__ void load(const F<ST>& f) {
+    s = as_global(f.s);
+    r = as_global(f.r);
+    pitch = f.pitch;
+    rows = f.rows;
+    cols = f.cols;
+  }
+  __device__ __forceinline__ gptr<ST> row(int i) const { return s + (int64_t)i * pitch; }
+  __device__ __forceinline__ gptr<ST> rrow(int i) const { return r + (int64_t)i * pitch; }
+};
+
+template <class ST, class CT>
+__device__ __forceinline__ void ldx_r(const FR<ST>& P, int row, int c, int periodic, int M, CT o[4]) {
+  if (periodic) row = row < 0 ? row + M : (row >= M ? row - M : row);
+  if (row < 0 || row >= P.rows || c >= P.cols) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) o[j] = (CT)0;
+    return;
+  }
+  ld4<ST, CT>(P.row(row), c, o);
+}
+
+template <class ST, class CT>
+__device__ __forceinline__ void seg_fill_r(CT* seg, const FR<ST>& P, int row, int base, int lane, int periodic, int M) {
+  F<ST> f;
+  f.s = (ST*)P.s;
+  f.r = (ST*)P.r;
+  f.c = nullptr;
+  f.rows = P.rows;
+  f.cols = P.cols;
+  f.pitch = P.pitch;
+  seg_fill<ST, CT>(seg, f, row, base, lane, periodic, M);
+}
+
+// 1/x in FP64 for the energy weights: hardware reciprocal refined by one
+// Newton step (explicit FMAs), within an ulp or two of the IEEE quotient;
+// energies are reductions compared at 1e-12 relative, not bitwise.
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  const double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  return r;
+}
+
+// energy term of a core point (same weights as energy_pt; core points are
+// interior along SBP axes, so the H factor there is dx)
+template <class CT>
+__device__ __forceinline__ double energy_core(int eform, double dx, CT v, CT prm, int tx, int ty, int r, int col, int rows,
+                                              int cols, int perx, int pery) {
+  if (eform == 1) {
+    double w = dx * dx / 2.;
+    if (ty && (col == 0 || col == cols - 1)) w /= 2.;
+    if (tx && (r == 0 || r == rows - 1)) w /= 2.;
+    return (double)((CT)((CT)(v * prm) * v)) * w;
+  }
+  const double hx = (perx && tx && r == rows - 1) ? 0.0 : dx;
+  const double hy = (pery && ty && col == cols - 1) ? 0.0 : dx;
+  const double comp = rcp64((double)prm);
+  const double w = (comp * hx) * hy;
+  const double s = (double)v;
+  return s * w * s;
+}
+
+template <class ST>
+__device__ __forceinline__ void store4(const FR<ST>& X, int r, int c, const V4<ST>& sn, const V4<ST>& rn, int rm, int full,
+                                       const bool in[4]) {
+  if (full) {
+    st_v4<ST>(X.row(r) + c, sn);
+    if (rm) st_v4<ST>(X.rrow(r) + c, rn);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (in[j]) {
+        st_1<ST>(X.row(r) + c + j, sn.v[j]);
+        if (rm) st_1<ST>(X.rrow(r) + c + j, rn.v[j]);
+      }
+  }
+}
+
+// Leapfrog update in registers with the compensation variant fixed at
+// compile time (acc already holds R or the residual where the variant keeps
+// one).  Same arithmetic as update().
+template <class ST, class CT, int CP>
+__device__ __forceinline__ void upd(ST& s, ST& r, CT acc) {
+  using S = St<ST>;
+  if constexpr (sizeof(ST) >= 4) {
+    const CT a = S::ld(s);
+    if constexpr (CP == TW_CPST_0) {
+      s = a + acc;
+    } else if constexpr (CP == TW_CPST_3R) {
+      const CT t = a + acc;
+      const CT z = t - a;
+      r = acc - z;
+      s = t;
+    } else if constexpr (CP == TW_CPST_M3R) {
+      const CT Y = acc;
+      const CT T = a + Y;
+      CT u = T - a;
+      u = Y - u;
+      r = u;
+      s = T;
+    } else {  // TW_CPST_6R
+      const CT t = a + acc;
+      const CT pa = t - acc;
+      const CT pb = t - pa;
+      const CT da = a - pa;
+      const CT db = acc - pb;
+      s = t;
+      r = da + db;
+    }
+  } else {
+    const CT t = S::ld(s) + acc;
+    const ST hi = S::st(t);
+    s = hi;
+    if constexpr (CP == TW_CPST_K) r = S::st(t - S::ld(hi));
+  }
+}
+
+template <int CP>
+struct CPT {
+  static constexpr bool rm = CP == TW_CPST_3R || CP == TW_CPST_M3R || CP == TW_CPST_6R || CP == TW_CPST_K;
+  static constexpr bool k = CP == TW_CPST_K;
+};
+
+// 12 consecutive segment values around the lane's 4 columns: out[i] = column c - 4 + i
+template <class CT>
+__device__ __forceinline__ void seg12(const CT* seg, int lane, CT out[12]) {
+  const CT* p = seg + HALO + lane * VW - 4;
+  if constexpr (sizeof(CT) == 4) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const float4 v = *(const float4*)(p + 4 * q);
+      out[4 * q + 0] = v.x;
+      out[4 * q + 1] = v.y;
+      out[4 * q + 2] = v.z;
+      out[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const double2 v = *(const double2*)(p + 2 * q);
+      out[2 * q + 0] = v.x;
+      out[2 * q + 1] = v.y;
+    }
+  }
+}
+
+// y derivative of column j (0..3) of the lane from seg12 values
+template <class CT>
+__device__ __forceinline__ CT dy12(const CT v[12], int j, int out_t, const CT st[4]) {
+  const int b = 4 + j + (out_t ? -2 : -1);
+  CT d = (CT)0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) d = d + v[b + k] * st[k];
+  return d;
+}
+
+// source on the one row / lane that holds it (uniform row test first)
+template <class ST, class CT, int CP>
+__device__ __forceinline__ void src4(V4<ST>& sn, V4<ST>& rn, const Dyn& D, int grid, int r, int c) {
+  if (!(D.src_on && D.src_grid == grid && r == D.src_ix)) return;
+  const int j = D.src_iy - c;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {  // selects only: no dynamic index into the register arrays
+    ST s1 = sn.v[q], r1 = rn.v[q];
+    add_source<ST, CT>(&s1, &r1, D.src_inc, CPT<CP>::k);
+    const bool hit = q == j;
+    sn.v[q] = hit ? s1 : sn.v[q];
+    rn.v[q] = hit ? r1 : rn.v[q];
+  }
+}
+
+// MM energy of one cell (grid_energy.cpp:10-27, 180-201), core point (H = dx along both axes)
+template <class CT>
+__device__ __forceinline__ double energy_mm_core(double dx, CT lam, CT tmu, CT sxx, CT syy) {
+  const double L = (double)lam, M2 = (double)tmu;
+  const double rd = rcp64(M2 * (2. * L + M2));
+  double p1 = (L + M2) * rd;
+  double p2 = -(L * 2.) * rd;
+  p1 = (p1 * dx) * dx;
+  p2 = (p2 * dx) * dx;
+  const double a = (double)sxx, b = (double)syy;
+  return a * p1 * a + b * p1 * b + a * p2 * b;
+}
+
+// segment fill for a strip whose halo lies inside the row (no wrap, no bounds)
+template <class ST, class CT>
+__device__ __forceinline__ void seg_fill_fast(CT* seg, const FR<ST>& P, int row, int base, int lane) {
+  const auto rp = P.row(row);
+  CT o[4];
+  ld4<ST, CT>(rp, base + lane * VW, o);
+#pragma unroll
+  for (int j = 0; j < 4; j++) seg[HALO + lane * VW + j] = o[j];
+  if (lane < 2) {
+    ld4<ST, CT>(rp, base - HALO + lane * 4, o);
+#pragma unroll
+    for (int j = 0; j < 4; j++) seg[lane * 4 + j] = o[j];
+  } else if (lane >= 62) {
+    ld4<ST, CT>(rp, base + STRIP + (lane - 62) * 4, o);
+#pragma unroll
+    for (int j = 0; j < 4; j++) seg[HALO + STRIP + (lane - 62) * 4 + j] = o[j];
+  }
+}
+
+template <bool FULL, class ST, class CT>
+__device__ __forceinline__ void ldx_f(const FR<ST>& P, int row, int c, int periodic, int M, CT o[4]) {
+  if constexpr (FULL) {
+    if (periodic) row = row < 0 ? row + M : (row >= M ? row - M : row);
+    if (row < 0 || row >= P.rows) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) o[j] = (CT)0;
+      return;
+    }
+    ld4<ST, CT>(P.row(row), c, o);
+  } else {
+    ldx_r<ST, CT>(P, row, c, periodic, M, o);
+  }
+}
+
+template <bool FULL, class ST, class CT>
+__device__ __forceinline__ void seg_f(CT* seg, const FR<ST>& P, int row, int base, int lane, int periodic, int M) {
+  if constexpr (FULL)
+    seg_fill_fast<ST, CT>(seg, P, row, base, lane);
+  else
+    seg_fill_r<ST, CT>(seg, P, row, base, lane, periodic, M);
+}
+
+template <bool FULL, class ST>
+__device__ __forceinline__ void store_f(const FR<ST>& X, int r, int c, const V4<ST>& sn, const V4<ST>& rn, bool rm,
+                                        int full, const bool in[4]) {
+  if constexpr (FULL) {
+    st_v4<ST>(X.row(r) + c, sn);
+    if (rm) st_v4<ST>(X.rrow(r) + c, rn);
+  } else {
+    store4<ST>(X, r, c, sn, rn, rm, full, in);
+  }
+}
+
+// Velocity half step, core region, streaming.
+template <class ST, class KT, class CT, int PH, int CP>
+__global__ void __launch_bounds__(64) k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+  constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
+  __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // sxy row (y partner of vx, elastic)
+  __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // syy or p row (y partner of vy)
+  __shared__ double sh[8];
+  const Stat<ST, KT, CT>& A = *Sp;
+  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, D, sh);
+  FR<ST> VX, VY, PX, PY, SXY;
+  VX.load(A.f[TW_VX]);
+  VY.load(A.f[TW_VY]);
+  PX.load(A.f[PH == TW_ELASTIC ? TW_SXX : TW_P]);
+  PY.load(A.f[PH == TW_ELASTIC ? TW_SYY : TW_P]);
+  if constexpr (PH == TW_ELASTIC) SXY.load(A.f[TW_SXY]);
+  const KT* bxp = A.k[TW_C_BINV_VX].p;
+  const int64_t bxs = A.k[TW_C_BINV_VX].pitch;
+  const KT* byp = A.k[TW_C_BINV_VY].p;
+  const int64_t bys = A.k[TW_C_BINV_VY].pitch;
+  CT st[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) st[k] = A.t.st[k];
+  const int perx = A.perx, pery = A.pery, mx = A.mx, my = A.my, eform = A.eform;
+  const double dx = A.dx;
+  const int energy = D.energy;
+
+  const int lane = threadIdx.x;
+  const int base = blockIdx.x * STRIP;
+  const int c = base + lane * VW;
+  const int x0 = C.rlo + blockIdx.y * C.txc;
+  const int x1 = min(x0 + C.txc, C.rhi);
+  const bool wave_full = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
+  double e = 0.0;
+
+  auto rows = [&](auto fullc) {
+    constexpr bool FULL = decltype(fullc)::value;
+    bool inx[4], iny[4];
+    int fullx = 1, fully = 1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int col = c + j;
+      inx[j] = FULL || (col >= C.clo && col < C.chi && col < VX.cols);
+      iny[j] = FULL || (col >= C.clo && col < C.chi && col < VY.cols);
+      fullx &= inx[j];
+      fully &= iny[j];
+    }
+    const bool anyx = inx[0] || inx[1] || inx[2] || inx[3];
+    const bool anyy = iny[0] || iny[1] || iny[2] || iny[3];
+    CT wa[4][4], wb[4][4];  // vx: x partner rows r-2..r+1 ; vy: sxy rows r-1..r+2
+#pragma unroll
+    for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(PX, x0 - 2 + k, c, perx, mx, wa[k + 1]);
+    if constexpr (PH == TW_ELASTIC) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(SXY, x0 - 1 + k, c, perx, mx, wb[k + 1]);
+    }
+    int buf = 0;
+    for (int r = x0; r < x1; r++) {
+      shift_win(wa);
+      ldx_f<FULL, ST, CT>(PX, r + 1, c, perx, mx, wa[3]);
+      if constexpr (PH == TW_ELASTIC) {
+        shift_win(wb);
+        ldx_f<FULL, ST, CT>(SXY, r + 2, c, perx, mx, wb[3]);
+        seg_f<FULL, ST, CT>(segA[buf], SXY, r, base, lane, pery, my);
+      }
+      const bool vy_row = r < VY.rows;
+      if (vy_row) seg_f<FULL, ST, CT>(segB[buf], PY, r, base, lane, pery, my);
+      __syncthreads();
+      if (anyx && r < VX.rows) {  // ---- vx (NM)
+        CT b[4];
+        ld4<KT, CT>(bxp + (int64_t)r * bxs, c, b);
+        V4<ST> sn = ld_v4<ST>(VX.row(r) + c);
+        V4<ST> rn = sn;
+        if constexpr (RM) rn = ld_v4<ST>(VX.rrow(r) + c);
+        CT ya[12];
+        if constexpr (PH == TW_ELASTIC) seg12<CT>(segA[buf], lane, ya);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+          acc = acc + dx_win<CT>(wa, j, st) * b[j];
+          if constexpr (PH == TW_ELASTIC) acc = acc + dy12<CT>(ya, j, 0, st) * b[j];
+          upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
+        }
+        src4<ST, CT, CP>(sn, rn, D, TW_GRID_NM, r, c);
+        if (energy) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const double t = energy_core<CT>(eform, dx, value_of<ST, CT>(sn.v[j], rn.v[j], KM), b[j], 1, 0, r, c + j,
+                                             VX.rows, VX.cols, perx, pery);
+            e += inx[j] ? t : 0.0;
+          }
+        }
+        store_f<FULL, ST>(VX, r, c, sn, rn, RM, fullx, inx);
+      }
+      if (anyy && vy_row) {  // ---- vy (MN)
+        CT b[4];
+        ld4<KT, CT>(byp + (int64_t)r * bys, c, b);
+        V4<ST> sn = ld_v4<ST>(VY.row(r) + c);
+        V4<ST> rn = sn;
+        if constexpr (RM) rn = ld_v4<ST>(VY.rrow(r) + c);
+        CT yb[12];
+        seg12<CT>(segB[buf], lane, yb);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+          if constexpr (PH == TW_ELASTIC) acc = acc + dx_win<CT>(wb, j, st) * b[j];
+          acc = acc + dy12<CT>(yb, j, 1, st) * b[j];
+          upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
+        }
+        src4<ST, CT, CP>(sn, rn, D, TW_GRID_MN, r, c);
+        if (energy) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const double t = energy_core<CT>(eform, dx, value_of<ST, CT>(sn.v[j], rn.v[j], KM), b[j], 0, 1, r, c + j,
+                                             VY.rows, VY.cols, perx, pery);
+            e += iny[j] ? t : 0.0;
+          }
+        }
+        store_f<FULL, ST>(VY, r, c, sn, rn, RM, fully, iny);
+      }
+      buf ^= 1;
+    }
+  };
+  if (wave_full)
+    rows(std::true_type{});
+  else
+    rows(std::false_type{});
+  if (energy) {
+    const double bsum = block_sum(e, sh);
+    if (lane == 0) D.part[blockIdx.y * gridDim.x + blockIdx.x] = bsum;
+  }
+}
+
+// Stress half step, core region, streaming.
+template <class ST, class KT, class CT, int PH, int CP>
+__global__ void __launch_bounds__(64) k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+  constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
+  __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // vx row (y partner of sxy)
+  __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // vy row (y partner of MM / p)
+  __shared__ double sh[8];
+  const Stat<ST, KT, CT>& A = *Sp;
+  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, D, sh);
+  FR<ST> VX, VY, XN, X0, X1;
+  VX.load(A.f[TW_VX]);
+  VY.load(A.f[TW_VY]);
+  X0.load(A.f[PH == TW_ELASTIC ? TW_SXX : TW_P]);
+  if constexpr (PH == TW_ELASTIC) {
+    X1.load(A.f[TW_SYY]);
+    XN.load(A.f[TW_SXY]);
+  }
+  const KT* k0p = A.k[PH == TW_ELASTIC ? TW_C_LAMBDA : TW_C_KAPPA].p;
+  const int64_t k0s = A.k[PH == TW_ELASTIC ? TW_C_LAMBDA : TW_C_KAPPA].pitch;
+  const KT* k1p = A.k[TW_C_TWOMU].p;
+  const int64_t k1s = A.k[TW_C_TWOMU].pitch;
+  const KT* kmp = A.k[TW_C_MU].p;
+  const int64_t kms = A.k[TW_C_MU].pitch;
+  CT st[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) st[k] = A.t.st[k];
+  const int perx = A.perx, pery = A.pery, mx = A.mx, my = A.my, eform = A.eform;
+  const double dx = A.dx;
+  const int energy = D.energy;
+
+  const int lane = threadIdx.x;
+  const int base = blockIdx.x * STRIP;
+  const int c = base + lane * VW;
+  const int x0 = C.rlo + blockIdx.y * C.txc;
+  const int x1 = min(x0 + C.txc, C.rhi);
+  const bool wave_full = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
+  double e = 0.0;
+
+  auto rows = [&](auto fullc) {
+    constexpr bool FULL = decltype(fullc)::value;
+    bool inn[4], inm[4];
+    int fulln = 1, fullm = 1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int col = c + j;
+      inn[j] = PH == TW_ELASTIC && (FULL || (col >= C.clo && col < C.chi && col < XN.cols));
+      inm[j] = FULL || (col >= C.clo && col < C.chi && col < X0.cols);
+      fulln &= inn[j];
+      fullm &= inm[j];
+    }
+    const bool anyn = inn[0] || inn[1] || inn[2] || inn[3];
+    const bool anym = inm[0] || inm[1] || inm[2] || inm[3];
+    CT wa[4][4], wb[4][4];  // sxy: vy rows r-2..r+1 ; MM: vx rows r-1..r+2
+    if constexpr (PH == TW_ELASTIC) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(VY, x0 - 2 + k, c, perx, mx, wa[k + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(VX, x0 - 1 + k, c, perx, mx, wb[k + 1]);
+    int buf = 0;
+    for (int r = x0; r < x1; r++) {
+      if constexpr (PH == TW_ELASTIC) {
+        shift_win(wa);
+        ldx_f<FULL, ST, CT>(VY, r + 1, c, perx, mx, wa[3]);
+        seg_f<FULL, ST, CT>(segA[buf], VX, r, base, lane, pery, my);
+      }
+      shift_win(wb);
+      ldx_f<FULL, ST, CT>(VX, r + 2, c, perx, mx, wb[3]);
+      const bool mm_row = r < X0.rows;
+      if (mm_row) seg_f<FULL, ST, CT>(segB[buf], VY, r, base, lane, pery, my);
+      __syncthreads();
+      if constexpr (PH == TW_ELASTIC) {
+        if (anyn && r < XN.rows) {  // ---- sxy (NN)
+          CT mu[4];
+          ld4<KT, CT>(kmp + (int64_t)r * kms, c, mu);
+          V4<ST> sn = ld_v4<ST>(XN.row(r) + c);
+          V4<ST> rn = sn;
+          if constexpr (RM) rn = ld_v4<ST>(XN.rrow(r) + c);
+          CT ya[12];
+          seg12<CT>(segA[buf], lane, ya);
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+            acc = acc + dx_win<CT>(wa, j, st) * mu[j];
+            acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
+            upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
+          }
+          src4<ST, CT, CP>(sn, rn, D, TW_GRID_NN, r, c);
+          if (energy) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const double t = energy_core<CT>(eform, dx, value_of<ST, CT>(sn.v[j], rn.v[j], KM), mu[j], 1, 1, r, c + j,
+                                               XN.rows, XN.cols, perx, pery);
+              e += inn[j] ? t : 0.0;
+            }
+          }
+          store_f<FULL, ST>(XN, r, c, sn, rn, RM, fulln, inn);
+        }
+      }
+      if (anym && mm_row) {
+        CT yb[12];
+        seg12<CT>(segB[buf], lane, yb);
+        if constexpr (PH == TW_ELASTIC) {  // ---- sxx, syy (MM)
+          CT lam[4], tmu[4];
+          ld4<KT, CT>(k0p + (int64_t)r * k0s, c, lam);
+          ld4<KT, CT>(k1p + (int64_t)r * k1s, c, tmu);
+          V4<ST> n0 = ld_v4<ST>(X0.row(r) + c);
+          V4<ST> n1 = ld_v4<ST>(X1.row(r) + c);
+          V4<ST> m0 = n0, m1 = n1;
+          if constexpr (RM) {
+            m0 = ld_v4<ST>(X0.rrow(r) + c);
+            m1 = ld_v4<ST>(X1.rrow(r) + c);
+          }
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            CT a0 = RM ? (CT)St<ST>::ld(m0.v[j]) : (CT)0;
+            CT a1 = RM ? (CT)St<ST>::ld(m1.v[j]) : (CT)0;
+            const CT v = dx_win<CT>(wb, j, st);
+            a0 = a0 + v * lam[j];
+            a1 = a1 + v * lam[j];
+            a0 = a0 + v * tmu[j];
+            const CT w = dy12<CT>(yb, j, 0, st);
+            a0 = a0 + w * lam[j];
+            a1 = a1 + w * lam[j];
+            a1 = a1 + w * tmu[j];
+            upd<ST, CT, CP>(n0.v[j], m0.v[j], a0);
+            upd<ST, CT, CP>(n1.v[j], m1.v[j], a1);
+          }
+          src4<ST, CT, CP>(n0, m0, D, TW_GRID_MM, r, c);
+          src4<ST, CT, CP>(n1, m1, D, TW_GRID_MM, r, c);
+          if (energy) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const double t = energy_mm_core<CT>(dx, lam[j], tmu[j], value_of<ST, CT>(n0.v[j], m0.v[j], KM),
+                                                  value_of<ST, CT>(n1.v[j], m1.v[j], KM));
+              e += inm[j] ? t : 0.0;
+            }
+          }
+          store_f<FULL, ST>(X0, r, c, n0, m0, RM, fullm, inm);
+          store_f<FULL, ST>(X1, r, c, n1, m1, RM, fullm, inm);
+        } else {  // ---- p (MM, acoustic)
+          CT kap[4];
+          ld4<KT, CT>(k0p + (int64_t)r * k0s, c, kap);
+          V4<ST> sn = ld_v4<ST>(X0.row(r) + c);
+          V4<ST> rn = sn;
+          if constexpr (RM) rn = ld_v4<ST>(X0.rrow(r) + c);
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+            acc = acc + dx_win<CT>(wb, j, st) * kap[j];
+            acc = acc + dy12<CT>(yb, j, 0, st) * kap[j];
+            upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
+          }
+          src4<ST, CT, CP>(sn, rn, D, TW_GRID_MM, r, c);
+          if (energy) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const double t = energy_core<CT>(eform, dx, value_of<ST, CT>(sn.v[j], rn.v[j], KM), kap[j], 0, 0, r, c + j,
+                                               X0.rows, X0.cols, perx, pery);
+              e += inm[j] ? t : 0.0;
+            }
+          }
+          store_f<FULL, ST>(X0, r, c, sn, rn, RM, fullm, inm);
+        }
+      }
+      buf ^= 1;
+    }
+  };
+  if (wave_full)
+    rows(std::true_type{});
+  else
+    rows(std::false_type{});
+  if (energy) {
+    const double bsum = block_sum(e, sh);
+    if (lane == 0) D.part[blockIdx.y * gridDim.x + blockIdx.x] = bsum;
+  }
+}
+
 // ------------------------------------------------------------ dispatch
+// Core region and boundary band of the streaming launch (see Core / Rects).
+static void core_geometry(const Plan& p, Core* C, Rects* R) {
+  C->rlo = p.perx ? 0 : 4;
+  C->rhi = p.perx ? p.mx + 1 : p.mx - 4;
+  C->clo = p.pery ? 0 : 4;
+  C->chi = p.pery ? p.my + 1 : p.my - 4;
+  C->txc = p.txc;
+  memset(R, 0, sizeof *R);
+  auto add = [&](int r0, int r1, int c0, int c1) {
+    const int k = R->n++;
+    R->r0[k] = r0;
+    R->r1[k] = r1;
+    R->c0[k] = c0;
+    R->c1[k] = c1;
+    R->cum[k + 1] = R->cum[k] + ((r1 - r0 + 3) / 4) * ((c1 - c0 + 63) / 64);
+  };
+  if (!p.perx) {
+    add(0, 4, 0, p.my + 1);
+    add(p.mx - 4, p.mx + 1, 0, p.my + 1);
+  }
+  if (!p.pery) {
+    add(C->rlo, C->rhi, 0, 4);
+    add(C->rlo, C->rhi, p.my - 4, p.my + 1);
+  }
+}
+
+static bool use_stream(const Plan& p) {
+  return p.kver != 1 && !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C);
+}
+
 void grid_dims(const Plan& p, dim3* grid, dim3* block) {
+  if (!use_stream(p)) {
+    *block = dim3(64, 4, 1);
+    *grid = dim3((unsigned)((p.my + 1 + 63) / 64), (unsigned)((p.mx + 1 + 3) / 4), 1);
+    return;
+  }
+  Core C;
+  Rects R;
+  core_geometry(p, &C, &R);
+  *block = dim3(64, 1, 1);
+  *grid = dim3((unsigned)((C.chi + STRIP - 1) / STRIP), (unsigned)((C.rhi - C.rlo + p.txc - 1) / p.txc), 1);
+}
+
+// energy partials written per half step (core blocks + band blocks)
+int half_parts(const Plan& p) {
+  dim3 g, b;
+  grid_dims(p, &g, &b);
+  if (!use_stream(p)) return (int)(g.x * g.y);
+  Core C;
+  Rects R;
+  core_geometry(p, &C, &R);
+  return (int)(g.x * g.y) + R.cum[R.n];
+}
+
+static void grid_dims_v1(const Plan& p, dim3* grid, dim3* block) {
   *block = dim3(64, 4, 1);
   *grid = dim3((unsigned)((p.my + 1 + 63) / 64), (unsigned)((p.mx + 1 + 3) / 4), 1);
 }
 
+int energy_parts(const Plan& p) {
+  dim3 g, b;
+  grid_dims_v1(p, &g, &b);
+  return (int)(g.x * g.y);
+}
+
+template <class ST, class KT, class CT, int PH, int CP>
+static void launch_core_t(int half, const Stat<ST, KT, CT>* sp, const Dyn& d, const Core& C, dim3 g, dim3 b,
+                          hipStream_t s) {
+  if (half == 0)
+    hipLaunchKernelGGL((k_vel3<ST, KT, CT, PH, CP>), g, b, 0, s, sp, d, C);
+  else
+    hipLaunchKernelGGL((k_str3<ST, KT, CT, PH, CP>), g, b, 0, s, sp, d, C);
+}
+
+template <class ST, class KT, class CT, int PH>
+static hipError_t launch_core_p(const Plan& p, int half, const Stat<ST, KT, CT>* sp, const Dyn& d, const Core& C,
+                                dim3 g, dim3 b, hipStream_t s) {
+  if constexpr (sizeof(ST) >= 4) {
+    switch (p.cpst) {
+      case TW_CPST_0: launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s); break;
+      case TW_CPST_3R: launch_core_t<ST, KT, CT, PH, TW_CPST_3R>(half, sp, d, C, g, b, s); break;
+      case TW_CPST_M3R: launch_core_t<ST, KT, CT, PH, TW_CPST_M3R>(half, sp, d, C, g, b, s); break;
+      case TW_CPST_6R: launch_core_t<ST, KT, CT, PH, TW_CPST_6R>(half, sp, d, C, g, b, s); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    if (p.cpst == TW_CPST_K)
+      launch_core_t<ST, KT, CT, PH, TW_CPST_K>(half, sp, d, C, g, b, s);
+    else
+      launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s);
+  }
+  return hipGetLastError();
+}
+
+template <class ST, class KT, class CT>
+static hipError_t launch_core(const Plan& p, int half, const Stat<ST, KT, CT>* sp, const Dyn& d, const Core& C, dim3 g,
+                              dim3 b, hipStream_t s) {
+  if (p.physics == TW_ELASTIC) return launch_core_p<ST, KT, CT, TW_ELASTIC>(p, half, sp, d, C, g, b, s);
+  return launch_core_p<ST, KT, CT, TW_ACOUSTIC>(p, half, sp, d, C, g, b, s);
+}
+
 template <class ST, class KT, class CT>
 static hipError_t run_half(const Plan& p, int half, const StepCtl& c, hipStream_t s) {
+  auto a = make_args<ST, KT, CT>(p, &c, half == 0 ? p.part_v : p.part_s);
+  if (!use_stream(p)) {
+    dim3 g, b;
+    grid_dims_v1(p, &g, &b);
+    Rects R;
+    memset(&R, 0, sizeof R);
+    if (half == 0)
+      hipLaunchKernelGGL((k_velocity<ST, KT, CT>), g, b, 0, s, a, R);
+    else
+      hipLaunchKernelGGL((k_stress<ST, KT, CT>), g, b, 0, s, a, R);
+    return hipGetLastError();
+  }
+  Core C;
+  Rects R;
+  core_geometry(p, &C, &R);
   dim3 g, b;
   grid_dims(p, &g, &b);
-  auto a = make_args<ST, KT, CT>(p, &c, half == 0 ? p.part_v : p.part_s);
-  if (half == 0)
-    hipLaunchKernelGGL((k_velocity<ST, KT, CT>), g, b, 0, s, a);
-  else
-    hipLaunchKernelGGL((k_stress<ST, KT, CT>), g, b, 0, s, a);
+  const Stat<ST, KT, CT>* sp = (const Stat<ST, KT, CT>*)p.dstat;
+  const Dyn d = a;
+  if (C.rhi > C.rlo && C.chi > C.clo) {
+    hipError_t e = launch_core<ST, KT, CT>(p, half, sp, d, C, g, b, s);
+    if (e != hipSuccess) return e;
+  } else {
+    // no core rows: the band launch carries the pending work
+    R.poff = 0;
+  }
+  if (R.n > 0) {
+    R.poff = (int32_t)(g.x * g.y);
+    Args<ST, KT, CT> ab = a;
+    if (C.rhi > C.rlo && C.chi > C.clo) ab.pend = Pending();  // done by the core launch
+    const int nb = R.cum[R.n];
+    if (half == 0)
+      hipLaunchKernelGGL((k_velocity<ST, KT, CT>), dim3(nb), dim3(64, 4), 0, s, ab, R);
+    else
+      hipLaunchKernelGGL((k_stress<ST, KT, CT>), dim3(nb), dim3(64, 4), 0, s, ab, R);
+  }
   return hipGetLastError();
+}
+
+template <class ST, class KT, class CT>
+static size_t stat_fill_t(const Plan& p, void* out) {
+  if (out) {
+    const Stat<ST, KT, CT> st = make_args<ST, KT, CT>(p, nullptr, nullptr);
+    memcpy(out, &st, sizeof st);
+  }
+  return sizeof(Stat<ST, KT, CT>);
+}
+
+size_t stat_fill(const Plan& p, void* out) {
+  switch (p.precision) {
+    case TW_FP64: return stat_fill_t<double, double, double>(p, out);
+    case TW_FP32: return stat_fill_t<float, float, float>(p, out);
+    case TW_BF16:
+      if (p.coef16) return stat_fill_t<bf16, bf16, float>(p, out);
+      return stat_fill_t<bf16, float, float>(p, out);
+    default:
+      if (p.coef16) return stat_fill_t<_Float16, _Float16, float>(p, out);
+      return stat_fill_t<_Float16, float, float>(p, out);
+  }
 }
 
 template <class ST, class KT, class CT>
 static hipError_t run_misc(const Plan& p, int what, const Pending* pend, hipStream_t s) {
   dim3 g, b;
-  grid_dims(p, &g, &b);
+  grid_dims_v1(p, &g, &b);
   StepCtl c;
   if (pend) c.pend = *pend;
   auto a = make_args<ST, KT, CT>(p, &c, p.part_s);
