@@ -319,7 +319,9 @@ void dfree(tw_solver* s, void** p) {
   *p = nullptr;
 }
 
-int64_t pitch_of(int cols) { return ((int64_t)cols + 63) / 64 * 64; }
+// row pitch in elements: columns rounded up to 64, plus an optional pad
+// (TW_PITCH_PAD, a multiple of 4 elements) that breaks power-of-two strides
+int64_t pitch_of(int cols, int pad) { return ((int64_t)cols + 63) / 64 * 64 + pad; }
 
 int upload_coef(tw_solver* s, int k) {
   const tw_config* c = &s->cfg;
@@ -509,6 +511,8 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   p.dx = c->dx;
   if (const char* kv = getenv("TW_KERNEL")) p.kver = atoi(kv) == 1 ? 1 : 2;
   if (const char* tx = getenv("TW_TXC")) p.txc = atoi(tx) > 0 ? atoi(tx) : 64;
+  int pad = 0;
+  if (const char* pp = getenv("TW_PITCH_PAD")) pad = atoi(pp) > 0 ? (atoi(pp) + 3) / 4 * 4 : 0;
   build_tables(c, &p.t);
   const size_t es = state_size(c->precision);
   auto bail = [&](int code) {
@@ -519,7 +523,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     if (!field_in(c->physics, f)) continue;
     tw::DevField& d = p.f[f];
     shape_of(c, field_grid(f), &d.rows, &d.cols);
-    d.pitch = pitch_of(d.cols);
+    d.pitch = pitch_of(d.cols, pad);
     const size_t bytes = (size_t)d.rows * d.pitch * es;
     if ((e = dalloc(s, &d.s, bytes))) return bail(e);
     if (rmode(c->cpst) && (e = dalloc(s, &d.r, bytes))) return bail(e);
@@ -530,7 +534,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     if (!coef_in(c->physics, k)) continue;
     tw::DevCoef& d = p.k[k];
     shape_of(c, coef_grid(k), &d.rows, &d.cols);
-    d.pitch = pitch_of(d.cols);
+    d.pitch = pitch_of(d.cols, pad);
     if ((e = dalloc(s, &d.p, (size_t)d.rows * d.pitch * ks))) return bail(e);
     s->hcoef[k].assign((size_t)d.rows * d.cols, 0.0);
   }

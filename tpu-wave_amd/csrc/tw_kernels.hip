@@ -1222,6 +1222,107 @@ __device__ __forceinline__ void store_f(const FR<ST>& X, int r, int c, const V4<
   }
 }
 
+// ---- helpers of the FULL-strip path (strip and its halos inside the row,
+// core rows only): x windows rotate through four register arrays, so the
+// row loop is unrolled by four and the windows are never copied.
+
+// row index along x with the periodic wrap (core rows never leave the grid
+// otherwise)
+__device__ __forceinline__ int wrap_row(int row, int periodic, int M) {
+  return periodic ? (row < 0 ? row + M : (row >= M ? row - M : row)) : row;
+}
+
+template <class ST, class CT>
+__device__ __forceinline__ void cvt_v4(const V4<ST>& v, CT (&o)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) o[j] = (CT)St<ST>::ld(v.v[j]);
+}
+
+// four converted values into a segment at p
+template <class ST, class CT>
+__device__ __forceinline__ void seg_put4(CT* p, const V4<ST>& v) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) p[j] = (CT)St<ST>::ld(v.v[j]);
+}
+
+// core-column mask of a lane's four columns; true when all four are in
+__device__ __forceinline__ bool col_mask(int c, int clo, int chi, bool (&in)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) in[j] = c + j >= clo && c + j < chi;
+  return in[0] && in[3];
+}
+
+// halo chunk column of the edge lanes (lanes 0-1 left, 62-63 right), clamped
+// inside the allocation; other lanes reload their own chunk.  Clamped halo
+// values only ever feed masked columns (core columns start at 4 along SBP y).
+__device__ __forceinline__ int halo_col(int lane, int base, int cl, int cmax) {
+  const int h = lane < 2 ? base - HALO + lane * 4 : (lane >= 62 ? base + STRIP + (lane - 62) * 4 : cl);
+  return h < 0 ? 0 : (h > cmax ? cmax : h);
+}
+
+// store of a lane's four values, element-wise where only some columns are in
+template <class ST, class PT>
+__device__ __forceinline__ void st_m(PT rowp, int c, const V4<ST>& v, bool full4, const bool (&in)[4]) {
+  if (full4) {
+    st_v4<ST>(rowp + c, v);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (in[j]) st_1<ST>(rowp + c + j, v.v[j]);
+  }
+}
+
+// x derivative of column j from the window rows in tap order (same operation
+// order as dx_win)
+template <class CT>
+__device__ __forceinline__ CT dx4(const CT (&w0)[4], const CT (&w1)[4], const CT (&w2)[4], const CT (&w3)[4], int j,
+                                  const CT st[4]) {
+  CT v = (CT)0;
+  v = v + w0[j] * st[0];
+  v = v + w1[j] * st[1];
+  v = v + w2[j] * st[2];
+  v = v + w3[j] * st[3];
+  return v;
+}
+
+// the lane's own four values of a segment row
+template <class CT>
+__device__ __forceinline__ void seg_own(CT* seg, const CT (&o)[4], int lane) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) seg[HALO + lane * VW + j] = o[j];
+}
+
+// n / d in FP64 for the energy terms: hardware reciprocal, quotient and one
+// residual correction (within an ulp or two of the IEEE quotient; energies are
+// sums compared at 1e-12 relative)
+__device__ __forceinline__ double div64(double n, double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  const double q = n * r;
+  const double e = __builtin_fma(-d, q, n);
+  return __builtin_fma(e, r, q);
+}
+
+// energy term of an interior point without its H weight dx*dx (applied to the
+// block sum): s^2 / coefficient (compliance 1/coefficient for velocities with
+// 1/rho, for sxy with mu and for p with kappa; grid_energy.cpp:8-216)
+template <class CT>
+__device__ __forceinline__ double e_single(CT v, CT prm) {
+  const double s = (double)v;
+  return div64(s * s, (double)prm);
+}
+
+// MM cell: [(L + M2)(a^2 + b^2) - 2 L a b] / (M2 (2 L + M2)), the quadratic
+// form of the 2x2 compliance with P1 = (L+M2)/(M2(2L+M2)), P2 = -2L/(M2(2L+M2))
+template <class CT>
+__device__ __forceinline__ double e_mm(CT lam, CT tmu, CT sxx, CT syy) {
+  const double L = (double)lam, M2 = (double)tmu;
+  const double a = (double)sxx, b = (double)syy;
+  const double q1 = __builtin_fma(a, a, b * b);
+  const double L2 = L + L;
+  const double n = __builtin_fma(L + M2, q1, -(L2 * (a * b)));
+  return div64(n, M2 * (L2 + M2));
+}
+
 // Velocity half step, core region, streaming.
 template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
@@ -1253,7 +1354,11 @@ __global__ void __launch_bounds__(64) k_vel3(const Stat<ST, KT, CT>* __restrict_
   const int c = base + lane * VW;
   const int x0 = C.rlo + blockIdx.y * C.txc;
   const int x1 = min(x0 + C.txc, C.rhi);
-  const bool wave_full = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
+  // fast path: every strip of an SBP y axis (columns masked to the core,
+  // loads clamped inside the allocation), interior strips of a periodic one
+  const bool inner = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
+  const bool wave_full = eform == 0 && (inner || !pery);
+  const int cmax = (my + 63) / 64 * 64 - 4;  // last 4-column chunk inside every row pitch
   double e = 0.0;
 
   auto rows = [&](auto fullc) {
@@ -1344,8 +1449,126 @@ __global__ void __launch_bounds__(64) k_vel3(const Stat<ST, KT, CT>* __restrict_
       buf ^= 1;
     }
   };
+  // FULL strips (energy form 0).  Every load of a row is issued before any of
+  // its stores, unconditionally and in bounds (halo addresses per lane,
+  // clamped rows), so a row costs one memory round trip; x windows rotate,
+  // segment own values come from the windows, and energy terms are summed
+  // unweighted and scaled by dx*dx once.
+  auto rows_full = [&]() {
+    bool in[4];
+    const bool full4 = col_mask(c, C.clo, C.chi, in);
+    const int cl = min(c, cmax);  // load column: inside the allocation for every lane
+    CT A0[4], A1[4], A2[4], A3[4];  // PX rows r-2..r+1 (x partner of vx)
+    CT B0[4], B1[4], B2[4], B3[4];  // SXY rows r-1..r+2 (x partner of vy), elastic
+    ldx_f<true, ST, CT>(PX, x0 - 2, cl, perx, mx, A0);
+    ldx_f<true, ST, CT>(PX, x0 - 1, cl, perx, mx, A1);
+    ldx_f<true, ST, CT>(PX, x0, cl, perx, mx, A2);
+    if constexpr (PH == TW_ELASTIC) {
+      ldx_f<true, ST, CT>(SXY, x0 - 1, cl, perx, mx, B0);
+      ldx_f<true, ST, CT>(SXY, x0, cl, perx, mx, B1);
+      ldx_f<true, ST, CT>(SXY, x0 + 1, cl, perx, mx, B2);
+    }
+    const int hc = halo_col(lane, base, cl, cmax);
+    const bool hl = lane < 2 || lane >= 62;
+    const int hs = lane < 2 ? lane * 4 : HALO + STRIP + (lane - 62) * 4;
+    double ef = 0.0;
+    auto body = [&](int r, int buf, CT(&a0)[4], CT(&a1)[4], CT(&a2)[4], CT(&a3)[4], CT(&b0)[4], CT(&b1)[4],
+                    CT(&b2)[4], CT(&b3)[4]) {
+      const bool vx_row = r < VX.rows, vy_row = r < VY.rows;
+      const int rx = vx_row ? r : VX.rows - 1, ry = vy_row ? r : VY.rows - 1;
+      // ---- loads
+      const V4<ST> nA = ld_v4<ST>(PX.row(wrap_row(r + 1, perx, mx)) + cl);
+      V4<ST> nB, hA, oB;
+      if constexpr (PH == TW_ELASTIC) {
+        nB = ld_v4<ST>(SXY.row(wrap_row(r + 2, perx, mx)) + cl);
+        hA = ld_v4<ST>(SXY.row(r) + hc);  // sxy row r halo
+        oB = ld_v4<ST>(PY.row(ry) + cl);   // syy row r
+      }
+      const V4<ST> hB = ld_v4<ST>(PY.row(ry) + hc);  // syy / p row r halo
+      const V4<KT> kx = ld_v4<KT>(bxp + (int64_t)rx * bxs + cl);
+      V4<ST> sx = ld_v4<ST>(VX.row(rx) + cl);
+      V4<ST> qx = sx;
+      if constexpr (RM) qx = ld_v4<ST>(VX.rrow(rx) + cl);
+      const V4<KT> ky = ld_v4<KT>(byp + (int64_t)ry * bys + cl);
+      V4<ST> sy = ld_v4<ST>(VY.row(ry) + cl);
+      V4<ST> qy = sy;
+      if constexpr (RM) qy = ld_v4<ST>(VY.rrow(ry) + cl);
+      // ---- segments
+      cvt_v4<ST, CT>(nA, a3);
+      if constexpr (PH == TW_ELASTIC) {
+        cvt_v4<ST, CT>(nB, b3);
+        if (perx && r >= mx) {  // duplicated node row: not the window's row
+          CT o[4];
+          ld4<ST, CT>(SXY.row(r), cl, o);
+          seg_own<CT>(segA[buf], o, lane);
+        } else {
+          seg_own<CT>(segA[buf], b1, lane);
+        }
+        if (hl) seg_put4<ST, CT>(segA[buf] + hs, hA);
+        CT o[4];
+        cvt_v4<ST, CT>(oB, o);
+        seg_own<CT>(segB[buf], o, lane);
+      } else {
+        seg_own<CT>(segB[buf], a2, lane);  // p row r
+      }
+      if (hl) seg_put4<ST, CT>(segB[buf] + hs, hB);
+      __syncthreads();
+      if (vx_row) {  // ---- vx (NM)
+        CT b[4];
+        cvt_v4<KT, CT>(kx, b);
+        CT ya[12];
+        if constexpr (PH == TW_ELASTIC) seg12<CT>(segA[buf], lane, ya);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          CT acc = RM ? (CT)St<ST>::ld(qx.v[j]) : (CT)0;
+          acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * b[j];
+          if constexpr (PH == TW_ELASTIC) acc = acc + dy12<CT>(ya, j, 0, st) * b[j];
+          upd<ST, CT, CP>(sx.v[j], qx.v[j], acc);
+        }
+        src4<ST, CT, CP>(sx, qx, D, TW_GRID_NM, r, c);
+        if (energy && !(perx && r == VX.rows - 1)) {  // the duplicated node row has weight 0
+#pragma unroll
+          for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sx.v[j], qx.v[j], KM), b[j]) : 0.0;
+        }
+        st_m<ST>(VX.row(r), c, sx, full4, in);
+        if constexpr (RM) st_m<ST>(VX.rrow(r), c, qx, full4, in);
+      }
+      if (vy_row) {  // ---- vy (MN)
+        CT b[4];
+        cvt_v4<KT, CT>(ky, b);
+        CT yb[12];
+        seg12<CT>(segB[buf], lane, yb);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          CT acc = RM ? (CT)St<ST>::ld(qy.v[j]) : (CT)0;
+          if constexpr (PH == TW_ELASTIC) acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * b[j];
+          acc = acc + dy12<CT>(yb, j, 1, st) * b[j];
+          upd<ST, CT, CP>(sy.v[j], qy.v[j], acc);
+        }
+        src4<ST, CT, CP>(sy, qy, D, TW_GRID_MN, r, c);
+        if (energy) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sy.v[j], qy.v[j], KM), b[j]) : 0.0;
+        }
+        st_m<ST>(VY.row(r), c, sy, full4, in);
+        if constexpr (RM) st_m<ST>(VY.rrow(r), c, qy, full4, in);
+      }
+    };
+    for (int r = x0;;) {
+      if (r >= x1) break;
+      body(r, 0, A0, A1, A2, A3, B0, B1, B2, B3);
+      if (++r >= x1) break;
+      body(r, 1, A1, A2, A3, A0, B1, B2, B3, B0);
+      if (++r >= x1) break;
+      body(r, 0, A2, A3, A0, A1, B2, B3, B0, B1);
+      if (++r >= x1) break;
+      body(r, 1, A3, A0, A1, A2, B3, B0, B1, B2);
+      ++r;
+    }
+    e = ef * (dx * dx);
+  };
   if (wave_full)
-    rows(std::true_type{});
+    rows_full();
   else
     rows(std::false_type{});
   if (energy) {
@@ -1389,7 +1612,11 @@ __global__ void __launch_bounds__(64) k_str3(const Stat<ST, KT, CT>* __restrict_
   const int c = base + lane * VW;
   const int x0 = C.rlo + blockIdx.y * C.txc;
   const int x1 = min(x0 + C.txc, C.rhi);
-  const bool wave_full = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
+  // fast path: every strip of an SBP y axis (columns masked to the core,
+  // loads clamped inside the allocation), interior strips of a periodic one
+  const bool inner = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
+  const bool wave_full = eform == 0 && (inner || !pery);
+  const int cmax = (my + 63) / 64 * 64 - 4;  // last 4-column chunk inside every row pitch
   double e = 0.0;
 
   auto rows = [&](auto fullc) {
@@ -1522,8 +1749,169 @@ __global__ void __launch_bounds__(64) k_str3(const Stat<ST, KT, CT>* __restrict_
       buf ^= 1;
     }
   };
+  // FULL strips (energy form 0): as in k_vel3, all loads of a row first,
+  // rotating windows, segment own values from the windows, unweighted
+  // energy terms scaled by dx*dx once.
+  auto rows_full = [&]() {
+    bool in[4];
+    const bool full4 = col_mask(c, C.clo, C.chi, in);
+    const int cl = min(c, cmax);  // load column: inside the allocation for every lane
+    CT A0[4], A1[4], A2[4], A3[4];  // VY rows r-2..r+1 (x partner of sxy), elastic
+    CT B0[4], B1[4], B2[4], B3[4];  // VX rows r-1..r+2 (x partner of MM)
+    if constexpr (PH == TW_ELASTIC) {
+      ldx_f<true, ST, CT>(VY, x0 - 2, cl, perx, mx, A0);
+      ldx_f<true, ST, CT>(VY, x0 - 1, cl, perx, mx, A1);
+      ldx_f<true, ST, CT>(VY, x0, cl, perx, mx, A2);
+    }
+    ldx_f<true, ST, CT>(VX, x0 - 1, cl, perx, mx, B0);
+    ldx_f<true, ST, CT>(VX, x0, cl, perx, mx, B1);
+    ldx_f<true, ST, CT>(VX, x0 + 1, cl, perx, mx, B2);
+    const int hc = halo_col(lane, base, cl, cmax);
+    const bool hl = lane < 2 || lane >= 62;
+    const int hs = lane < 2 ? lane * 4 : HALO + STRIP + (lane - 62) * 4;
+    double ef = 0.0;
+    auto body = [&](int r, int buf, CT(&a0)[4], CT(&a1)[4], CT(&a2)[4], CT(&a3)[4], CT(&b0)[4], CT(&b1)[4],
+                    CT(&b2)[4], CT(&b3)[4]) {
+      const bool mm_row = r < X0.rows;
+      const int rm = mm_row ? r : X0.rows - 1;
+      // ---- loads
+      V4<ST> nA, hA, sn, qn, oB;
+      V4<KT> kn;
+      if constexpr (PH == TW_ELASTIC) {
+        nA = ld_v4<ST>(VY.row(wrap_row(r + 1, perx, mx)) + cl);
+        hA = ld_v4<ST>(VX.row(r) + hc);  // vx row r halo
+        kn = ld_v4<KT>(kmp + (int64_t)r * kms + cl);
+        sn = ld_v4<ST>(XN.row(r) + cl);
+        qn = sn;
+        if constexpr (RM) qn = ld_v4<ST>(XN.rrow(r) + cl);
+      } else {
+        oB = ld_v4<ST>(VY.row(rm) + cl);  // vy row r (no window in the acoustic case)
+      }
+      const V4<ST> nB = ld_v4<ST>(VX.row(wrap_row(r + 2, perx, mx)) + cl);
+      const V4<ST> hB = ld_v4<ST>(VY.row(rm) + hc);  // vy row r halo
+      const V4<KT> k0 = ld_v4<KT>(k0p + (int64_t)rm * k0s + cl);
+      V4<KT> k1;
+      if constexpr (PH == TW_ELASTIC) k1 = ld_v4<KT>(k1p + (int64_t)rm * k1s + cl);
+      V4<ST> s0 = ld_v4<ST>(X0.row(rm) + cl), s1;
+      V4<ST> q0 = s0, q1;
+      if constexpr (RM) q0 = ld_v4<ST>(X0.rrow(rm) + cl);
+      if constexpr (PH == TW_ELASTIC) {
+        s1 = ld_v4<ST>(X1.row(rm) + cl);
+        q1 = s1;
+        if constexpr (RM) q1 = ld_v4<ST>(X1.rrow(rm) + cl);
+      }
+      // ---- segments
+      cvt_v4<ST, CT>(nB, b3);
+      if constexpr (PH == TW_ELASTIC) {
+        cvt_v4<ST, CT>(nA, a3);
+        if (perx && r >= mx) {  // duplicated node row of vx: not the window's row
+          CT o[4];
+          ld4<ST, CT>(VX.row(r), cl, o);
+          seg_own<CT>(segA[buf], o, lane);
+        } else {
+          seg_own<CT>(segA[buf], b1, lane);
+        }
+        if (hl) seg_put4<ST, CT>(segA[buf] + hs, hA);
+        seg_own<CT>(segB[buf], a2, lane);  // vy row r
+      } else {
+        CT o[4];
+        cvt_v4<ST, CT>(oB, o);
+        seg_own<CT>(segB[buf], o, lane);
+      }
+      if (hl) seg_put4<ST, CT>(segB[buf] + hs, hB);
+      __syncthreads();
+      if constexpr (PH == TW_ELASTIC) {  // ---- sxy (NN); every core row is an NN row
+        CT mu[4];
+        cvt_v4<KT, CT>(kn, mu);
+        CT ya[12];
+        seg12<CT>(segA[buf], lane, ya);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          CT acc = RM ? (CT)St<ST>::ld(qn.v[j]) : (CT)0;
+          acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * mu[j];
+          acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
+          upd<ST, CT, CP>(sn.v[j], qn.v[j], acc);
+        }
+        src4<ST, CT, CP>(sn, qn, D, TW_GRID_NN, r, c);
+        if (energy && !(perx && r == XN.rows - 1)) {  // the duplicated node row has weight 0
+#pragma unroll
+          for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sn.v[j], qn.v[j], KM), mu[j]) : 0.0;
+        }
+        st_m<ST>(XN.row(r), c, sn, full4, in);
+        if constexpr (RM) st_m<ST>(XN.rrow(r), c, qn, full4, in);
+      }
+      if (mm_row) {
+        CT yb[12];
+        seg12<CT>(segB[buf], lane, yb);
+        if constexpr (PH == TW_ELASTIC) {  // ---- sxx, syy (MM)
+          CT lam[4], tmu[4];
+          cvt_v4<KT, CT>(k0, lam);
+          cvt_v4<KT, CT>(k1, tmu);
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            CT a0 = RM ? (CT)St<ST>::ld(q0.v[j]) : (CT)0;
+            CT a1 = RM ? (CT)St<ST>::ld(q1.v[j]) : (CT)0;
+            const CT v = dx4<CT>(b0, b1, b2, b3, j, st);
+            a0 = a0 + v * lam[j];
+            a1 = a1 + v * lam[j];
+            a0 = a0 + v * tmu[j];
+            const CT w = dy12<CT>(yb, j, 0, st);
+            a0 = a0 + w * lam[j];
+            a1 = a1 + w * lam[j];
+            a1 = a1 + w * tmu[j];
+            upd<ST, CT, CP>(s0.v[j], q0.v[j], a0);
+            upd<ST, CT, CP>(s1.v[j], q1.v[j], a1);
+          }
+          src4<ST, CT, CP>(s0, q0, D, TW_GRID_MM, r, c);
+          src4<ST, CT, CP>(s1, q1, D, TW_GRID_MM, r, c);
+          if (energy) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+              ef += in[j] ? e_mm<CT>(lam[j], tmu[j], value_of<ST, CT>(s0.v[j], q0.v[j], KM),
+                                     value_of<ST, CT>(s1.v[j], q1.v[j], KM))
+                          : 0.0;
+          }
+          st_m<ST>(X0.row(r), c, s0, full4, in);
+          st_m<ST>(X1.row(r), c, s1, full4, in);
+          if constexpr (RM) {
+            st_m<ST>(X0.rrow(r), c, q0, full4, in);
+            st_m<ST>(X1.rrow(r), c, q1, full4, in);
+          }
+        } else {  // ---- p (MM, acoustic)
+          CT kap[4];
+          cvt_v4<KT, CT>(k0, kap);
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            CT acc = RM ? (CT)St<ST>::ld(q0.v[j]) : (CT)0;
+            acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * kap[j];
+            acc = acc + dy12<CT>(yb, j, 0, st) * kap[j];
+            upd<ST, CT, CP>(s0.v[j], q0.v[j], acc);
+          }
+          src4<ST, CT, CP>(s0, q0, D, TW_GRID_MM, r, c);
+          if (energy) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(s0.v[j], q0.v[j], KM), kap[j]) : 0.0;
+          }
+          st_m<ST>(X0.row(r), c, s0, full4, in);
+          if constexpr (RM) st_m<ST>(X0.rrow(r), c, q0, full4, in);
+        }
+      }
+    };
+    for (int r = x0;;) {
+      if (r >= x1) break;
+      body(r, 0, A0, A1, A2, A3, B0, B1, B2, B3);
+      if (++r >= x1) break;
+      body(r, 1, A1, A2, A3, A0, B1, B2, B3, B0);
+      if (++r >= x1) break;
+      body(r, 0, A2, A3, A0, A1, B2, B3, B0, B1);
+      if (++r >= x1) break;
+      body(r, 1, A3, A0, A1, A2, B3, B0, B1, B2);
+      ++r;
+    }
+    e = ef * (dx * dx);
+  };
   if (wave_full)
-    rows(std::true_type{});
+    rows_full();
   else
     rows(std::false_type{});
   if (energy) {
