@@ -68,7 +68,7 @@ struct Plan {
   int32_t energy_form;
   int32_t kver = 2;   // 1: per-point kernels, 2: streaming kernels
   void* dstat = nullptr;  // device copy of the static kernel arguments
-  int32_t txc = 64;   // rows per chunk of the streaming kernels
+  int32_t txc = 16;   // rows per chunk of the streaming kernels (TW_TXC overrides)
   double dx;
   Tables t;
   DevField f[NF];

@@ -507,14 +507,19 @@ __device__ void do_pending(const Stat<ST, KT, CT>& A, const Dyn& D, double* sh) 
 // ------------------------------------------------------------ half steps
 // Rectangles of the union index space covered by a per-point launch.  n = 0:
 // the whole (mx+1) x (my+1) space on a 2-D grid.  Otherwise blockIdx.x runs
-// over the 64x4 tiles of the n rectangles in order (the SBP boundary band
-// around the core region of the streaming kernels).
+// over the tiles of the n rectangles in order (the SBP boundary band around
+// the core region of the streaming kernels).  A tile is 256 points: tw[k]
+// columns by 256 / tw[k] rows, tw = 4 or 8 for the narrow column bands so no
+// thread of a block idles, 64 otherwise.
 struct Rects {
   int32_t n;
   int32_t r0[4], r1[4], c0[4], c1[4];
+  int32_t tw[4];   // tile width (columns) of each rectangle
   int32_t cum[5];  // first tile of each rectangle
   int32_t poff;    // offset of this launch's energy partials
 };
+
+__host__ __device__ inline int tile_width(int cols) { return cols <= 4 ? 4 : (cols <= 8 ? 8 : 64); }
 
 __device__ __forceinline__ bool map_point(const Rects& R, int* ix, int* iy, int* pidx) {
   if (R.n == 0) {
@@ -527,9 +532,11 @@ __device__ __forceinline__ bool map_point(const Rects& R, int* ix, int* iy, int*
   int k = 0;
   while (k + 1 < R.n && b >= R.cum[k + 1]) k++;
   const int t = b - R.cum[k];
-  const int tc = (R.c1[k] - R.c0[k] + 63) / 64;
-  *ix = R.r0[k] + (t / tc) * 4 + threadIdx.y;
-  *iy = R.c0[k] + (t % tc) * 64 + threadIdx.x;
+  const int tw = R.tw[k], th = 256 / tw;
+  const int tc = (R.c1[k] - R.c0[k] + tw - 1) / tw;
+  const int lin = threadIdx.y * blockDim.x + threadIdx.x;
+  *ix = R.r0[k] + (t / tc) * th + lin / tw;
+  *iy = R.c0[k] + (t % tc) * tw + lin % tw;
   *pidx = R.poff + b;
   return *ix < R.r1[k] && *iy < R.c1[k];
 }
@@ -1222,6 +1229,12 @@ __device__ __forceinline__ void store_f(const FR<ST>& X, int r, int c, const V4<
   }
 }
 
+// waves per SIMD the core kernels are compiled for: four (at most 128
+// VGPRs) for 16-bit storage and plain FP32, where latency hiding pays; FP32
+// with a residual array three; FP64 keeps its registers
+template <class ST, int CP>
+constexpr int WPE = sizeof(ST) == 8 ? 1 : (sizeof(ST) == 4 && CP != TW_CPST_0 ? 3 : 4);
+
 // ---- helpers of the FULL-strip path (strip and its halos inside the row,
 // core rows only): x windows rotate through four register arrays, so the
 // row loop is unrolled by four and the windows are never copied.
@@ -1325,7 +1338,7 @@ __device__ __forceinline__ double e_mm(CT lam, CT tmu, CT sxx, CT syy) {
 
 // Velocity half step, core region, streaming.
 template <class ST, class KT, class CT, int PH, int CP>
-__global__ void __launch_bounds__(64) k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>))) k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
   constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // sxy row (y partner of vx, elastic)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // syy or p row (y partner of vy)
@@ -1579,7 +1592,7 @@ __global__ void __launch_bounds__(64) k_vel3(const Stat<ST, KT, CT>* __restrict_
 
 // Stress half step, core region, streaming.
 template <class ST, class KT, class CT, int PH, int CP>
-__global__ void __launch_bounds__(64) k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>))) k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
   constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // vx row (y partner of sxy)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // vy row (y partner of MM / p)
@@ -1935,7 +1948,9 @@ static void core_geometry(const Plan& p, Core* C, Rects* R) {
     R->r1[k] = r1;
     R->c0[k] = c0;
     R->c1[k] = c1;
-    R->cum[k + 1] = R->cum[k] + ((r1 - r0 + 3) / 4) * ((c1 - c0 + 63) / 64);
+    const int tw = tile_width(c1 - c0), th = 256 / tw;
+    R->tw[k] = tw;
+    R->cum[k + 1] = R->cum[k] + ((r1 - r0 + th - 1) / th) * ((c1 - c0 + tw - 1) / tw);
   };
   if (!p.perx) {
     add(0, 4, 0, p.my + 1);
