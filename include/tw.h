@@ -214,6 +214,28 @@ int tw_set_stream(tw_solver* s, void* hip_stream);
 /* Check the state for non-finite values (TW_ENONFINITE if any). */
 int tw_check_finite(tw_solver* s);
 
+/* ---- discrete adjoint (acoustic; FP64 or FP32 storage; no compensation) ----
+ *
+ * Gradient of a receiver misfit L(traces) with respect to the material
+ * coefficients, by the transposed time step run backwards with sqrt(N)-step
+ * checkpointing and forward recomputation.  Stands in for the reference's
+ * gradient path of the inversion driver (cuda/src/cpp/inverse.cpp,
+ * cuda/src/cpp/forward.cpp:414-445: misfit sum (RESULT - record)^2).
+ *
+ * tw_adjoint_forward: advance nsteps from the current state like tw_step,
+ * keeping a checkpoint every `interval` steps (<= 0: ceil(sqrt(nsteps))).
+ * Receiver traces and energies are recorded as by tw_step. */
+int tw_adjoint_forward(tw_solver* s, int32_t nsteps, int32_t interval);
+
+/* tw_adjoint_run: given rcv_adjoint[n][nsteps] = dL/d(trace value) for the
+ * receivers of the last tw_adjoint_forward run (trace units as returned by
+ * tw_read_receivers divided by source_scaling), write dL/d(coefficient) as
+ * dense [rows][cols] doubles on each coefficient's grid: kappa (MM), 1/rho on
+ * NM, 1/rho on MN (null pointers skipped).  The fields are left as after the
+ * forward run. */
+int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, double* grad_binv_vx,
+                   double* grad_binv_vy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,8 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -285,6 +287,16 @@ struct tw_solver {
   int it = 0;
   bool ep_valid = false;
   bool need_uhat = false;
+  // discrete adjoint: checkpoints of the last checkpointed forward run
+  tw::AdjPlan adj{};
+  bool adj_taps = false;
+  void* adj_ckpt = nullptr;  // [nckpt] states
+  void* adj_seg = nullptr;   // [interval + 1] states
+  void* adj_final = nullptr; // state after the run
+  double* adj_inj = nullptr;
+  size_t adj_state_bytes = 0, adj_inj_n = 0;
+  int adj_it0 = 0, adj_n = 0, adj_I = 0, adj_nckpt = 0, adj_cap_ckpt = 0, adj_cap_seg = 0;
+  bool adj_done = false;     // a checkpointed run is available
 };
 
 namespace {
@@ -1003,3 +1015,276 @@ int tw_check_finite(tw_solver* s) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ discrete adjoint
+namespace {
+
+const int kAdjFields[3] = {TW_VX, TW_VY, TW_P};
+
+// forward taps along an axis of M cells (the per-point kernels' ddx/ddy):
+// out_t 1 = N output (M + 1 points, partner M), 0 = M output (partner N)
+void fwd_taps(const tw::Tables& t, int M, bool periodic, int out_t, std::vector<int>& nt,
+              std::vector<std::array<int, 5>>& idx, std::vector<std::array<double, 5>>& cf) {
+  const int n = out_t ? M + 1 : M;
+  nt.assign(n, 0);
+  idx.assign(n, {});
+  cf.assign(n, {});
+  const int sh0 = out_t ? -2 : -1, nb = out_t ? 4 : 3, np = out_t ? M : M + 1;
+  for (int i = 0; i < n; i++) {
+    if (periodic) {
+      nt[i] = 4;
+      for (int k = 0; k < 4; k++) {
+        idx[i][k] = ((i + sh0 + k) % M + M) % M;
+        cf[i][k] = t.st[k];
+      }
+    } else if (i < nb) {
+      nt[i] = 5;
+      for (int w = 0; w < 5; w++) {
+        idx[i][w] = w;
+        cf[i][w] = out_t ? t.DN_L[i][w] : t.DM_L[i][w];
+      }
+    } else if (i >= n - nb) {
+      const int r = i - (n - nb);
+      nt[i] = 5;
+      for (int w = 0; w < 5; w++) {
+        idx[i][w] = np - 5 + w;
+        cf[i][w] = out_t ? t.DN_R[r][w] : t.DM_R[r][w];
+      }
+    } else {
+      nt[i] = 4;
+      for (int k = 0; k < 4; k++) {
+        idx[i][k] = i + sh0 + k;
+        cf[i][k] = t.st[k];
+      }
+    }
+  }
+}
+
+// transpose of the taps, uploaded as a TapList (coefficients in the
+// arithmetic type)
+int upload_transpose(tw_solver* s, const tw::Tables& t, int M, bool periodic, int out_t, tw::TapList* L) {
+  std::vector<int> nt;
+  std::vector<std::array<int, 5>> idx;
+  std::vector<std::array<double, 5>> cf;
+  fwd_taps(t, M, periodic, out_t, nt, idx, cf);
+  const int np = out_t ? M : M + 1;
+  std::vector<int32_t> tn(np, 0), ti((size_t)np * tw::TMAX, 0);
+  std::vector<double> tc((size_t)np * tw::TMAX, 0.0);
+  for (int i = 0; i < (int)nt.size(); i++)
+    for (int k = 0; k < nt[i]; k++) {
+      const int j = idx[i][k];
+      if (tn[j] >= tw::TMAX) return fail(&s->err, TW_EINVAL, "adjoint tap list overflow");
+      ti[(size_t)j * tw::TMAX + tn[j]] = i;
+      tc[(size_t)j * tw::TMAX + tn[j]] = cf[i][k];
+      tn[j]++;
+    }
+  const size_t as = arith_size(s->cfg.precision);
+  int e;
+  if ((e = dalloc(s, (void**)&L->n, tn.size() * 4))) return e;
+  if ((e = dalloc(s, (void**)&L->idx, ti.size() * 4))) return e;
+  if ((e = dalloc(s, &L->cf, tc.size() * as))) return e;
+  HIPCHK(s, hipMemcpy(L->n, tn.data(), tn.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(s, hipMemcpy(L->idx, ti.data(), ti.size() * 4, hipMemcpyHostToDevice));
+  if (as == 8) {
+    HIPCHK(s, hipMemcpy(L->cf, tc.data(), tc.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(tc.begin(), tc.end());
+    HIPCHK(s, hipMemcpy(L->cf, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return TW_OK;
+}
+
+int adj_check(tw_solver* s) {
+  const tw_config* c = &s->cfg;
+  if (c->physics != TW_ACOUSTIC) return fail(&s->err, TW_EINVAL, "adjoint: acoustic physics only");
+  if (c->precision != TW_FP64 && c->precision != TW_FP32) return fail(&s->err, TW_EINVAL, "adjoint: FP64 or FP32 storage only");
+  if (c->cpst != TW_CPST_0) return fail(&s->err, TW_EINVAL, "adjoint: no compensation (cpst 0) only");
+  return TW_OK;
+}
+
+int adj_setup(tw_solver* s) {
+  if (s->adj_taps) return TW_OK;
+  Plan& p = s->plan;
+  const size_t as = arith_size(s->cfg.precision);
+  int e;
+  for (int f : kAdjFields) {
+    const tw::DevField& d = p.f[f];
+    if ((e = dalloc(s, &s->adj.mu[f], (size_t)d.rows * d.pitch * as))) return e;
+  }
+  const int gf[3] = {TW_P, TW_VX, TW_VY};
+  for (int i = 0; i < 3; i++) {
+    const tw::DevField& d = p.f[gf[i]];
+    if ((e = dalloc(s, (void**)&s->adj.g[i], (size_t)d.rows * d.pitch * 8))) return e;
+  }
+  for (int side = 0; side < 4; side++) {  // normal velocity of each side: vx on x sides, vy on y sides
+    const tw::DevField& d = p.f[side < 2 ? TW_VX : TW_VY];
+    if ((e = dalloc(s, (void**)&s->adj.gz[side], (size_t)(side < 2 ? d.cols : d.rows) * 8))) return e;
+  }
+  if ((e = upload_transpose(s, p.t, p.mx, p.perx, 1, &s->adj.gt[0]))) return e;
+  if ((e = upload_transpose(s, p.t, p.my, p.pery, 1, &s->adj.gt[1]))) return e;
+  if ((e = upload_transpose(s, p.t, p.mx, p.perx, 0, &s->adj.dt[0]))) return e;
+  if ((e = upload_transpose(s, p.t, p.my, p.pery, 0, &s->adj.dt[1]))) return e;
+  s->adj_state_bytes = 0;
+  for (int f : kAdjFields) s->adj_state_bytes += (size_t)p.f[f].rows * p.f[f].pitch * state_size(s->cfg.precision);
+  s->adj_taps = true;
+  return TW_OK;
+}
+
+// copy the live fields to / from a state slot
+int state_copy(tw_solver* s, void* slot, bool save) {
+  Plan& p = s->plan;
+  char* q = (char*)slot;
+  const size_t es = state_size(s->cfg.precision);
+  for (int f : kAdjFields) {
+    const size_t b = (size_t)p.f[f].rows * p.f[f].pitch * es;
+    if (save)
+      HIPCHK(s, hipMemcpyAsync(q, p.f[f].s, b, hipMemcpyDeviceToDevice, s->stream));
+    else
+      HIPCHK(s, hipMemcpyAsync(p.f[f].s, q, b, hipMemcpyDeviceToDevice, s->stream));
+    q += b;
+  }
+  return TW_OK;
+}
+
+tw::StateView state_view(tw_solver* s, void* slot) {
+  tw::StateView v;
+  char* q = (char*)slot;
+  const size_t es = state_size(s->cfg.precision);
+  for (int f : kAdjFields) {
+    v.s[f] = q;
+    q += (size_t)s->plan.f[f].rows * s->plan.f[f].pitch * es;
+  }
+  return v;
+}
+
+int grow(tw_solver* s, void** buf, int* cap, int want, size_t unit) {
+  if (*cap >= want) return TW_OK;
+  if (*buf) {
+    (void)hipStreamSynchronize(s->stream);
+    dfree(s, buf);
+  }
+  const int e = dalloc(s, buf, unit * (size_t)want);
+  if (e) return e;
+  *cap = want;
+  return TW_OK;
+}
+
+}  // namespace
+
+int tw_adjoint_forward(tw_solver* s, int32_t nsteps, int32_t interval) {
+  if (!s || nsteps <= 0) return fail(s ? &s->err : nullptr, TW_EINVAL, "bad argument");
+  (void)hipSetDevice(s->cfg.device);
+  int e;
+  if ((e = adj_check(s)) || (e = adj_setup(s))) return e;
+  if (interval <= 0) interval = (int)std::ceil(std::sqrt((double)nsteps));
+  if (interval > nsteps) interval = nsteps;
+  const int nck = (nsteps + interval - 1) / interval;
+  const size_t sb = s->adj_state_bytes;
+  if ((e = grow(s, &s->adj_ckpt, &s->adj_cap_ckpt, nck, sb))) return e;
+  if ((e = grow(s, &s->adj_seg, &s->adj_cap_seg, interval + 1, sb))) return e;
+  if (!s->adj_final && (e = dalloc(s, &s->adj_final, sb))) return e;
+  s->adj_it0 = s->it;
+  s->adj_n = nsteps;
+  s->adj_I = interval;
+  s->adj_nckpt = nck;
+  for (int k = 0; k < nsteps; k++) {
+    if (k % interval == 0 && (e = state_copy(s, (char*)s->adj_ckpt + sb * (size_t)(k / interval), true))) return e;
+    if ((e = tw_step(s, 1))) return e;
+  }
+  if ((e = state_copy(s, s->adj_final, true))) return e;
+  s->adj_done = true;
+  return TW_OK;
+}
+
+int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, double* grad_binv_vx,
+                   double* grad_binv_vy) {
+  if (!s || !rcv_adjoint) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  if (!s->adj_done) return fail(&s->err, TW_ESTATE, "tw_adjoint_run needs a tw_adjoint_forward run first");
+  (void)hipSetDevice(s->cfg.device);
+  Plan& p = s->plan;
+  const int n = s->adj_n, I = s->adj_I, it0 = s->adj_it0;
+  const size_t sb = s->adj_state_bytes;
+  int e;
+  // receiver sources
+  const size_t ninj = (size_t)(s->nrcv > 0 ? s->nrcv : 1) * n;
+  if (s->adj_inj_n < ninj) {
+    if (s->adj_inj) dfree(s, (void**)&s->adj_inj);
+    if ((e = dalloc(s, (void**)&s->adj_inj, ninj * 8))) return e;
+    s->adj_inj_n = ninj;
+  }
+  if (s->nrcv > 0)
+    HIPCHK(s, hipMemcpyAsync(s->adj_inj, rcv_adjoint, (size_t)s->nrcv * n * 8, hipMemcpyHostToDevice, s->stream));
+  s->adj.inj = s->adj_inj;
+  s->adj.nt = n;
+  const size_t as = arith_size(s->cfg.precision);
+  for (int f : kAdjFields) HIPCHK(s, hipMemsetAsync(s->adj.mu[f], 0, (size_t)p.f[f].rows * p.f[f].pitch * as, s->stream));
+  const int gf[3] = {TW_P, TW_VX, TW_VY};
+  for (int i = 0; i < 3; i++)
+    HIPCHK(s, hipMemsetAsync(s->adj.g[i], 0, (size_t)p.f[gf[i]].rows * p.f[gf[i]].pitch * 8, s->stream));
+  for (int side = 0; side < 4; side++) {
+    const tw::DevField& d = p.f[side < 2 ? TW_VX : TW_VY];
+    HIPCHK(s, hipMemsetAsync(s->adj.gz[side], 0, (size_t)(side < 2 ? d.cols : d.rows) * 8, s->stream));
+  }
+  for (int k = s->adj_nckpt - 1; k >= 0; k--) {
+    const int b = k * I, en = std::min(n, b + I);
+    // recompute the segment's states: slot t - b + 1 holds the state after step t
+    if ((e = state_copy(s, (char*)s->adj_ckpt + sb * (size_t)k, false))) return e;
+    s->it = it0 + b;
+    s->ep_valid = false;
+    if ((e = state_copy(s, s->adj_seg, true))) return e;
+    for (int t = b; t < en; t++) {
+      if ((e = tw_step(s, 1))) return e;
+      if ((e = state_copy(s, (char*)s->adj_seg + sb * (size_t)(t - b + 1), true))) return e;
+    }
+    for (int t = en - 1; t >= b; t--) {
+      const tw::StateView cur = state_view(s, (char*)s->adj_seg + sb * (size_t)(t - b + 1));
+      const tw::StateView prev = state_view(s, (char*)s->adj_seg + sb * (size_t)(t - b));
+      HIPCHK(s, tw::launch_adjoint_step(p, s->adj, cur, prev, it0 + t, t, s->stream));
+    }
+  }
+  // back to the state after the forward run
+  if ((e = state_copy(s, s->adj_final, false))) return e;
+  s->it = it0 + n;
+  s->ep_valid = false;
+  std::vector<double> g[3];
+  for (int i = 0; i < 3; i++) {
+    const tw::DevField& d = p.f[gf[i]];
+    g[i].resize((size_t)d.rows * d.cols);
+    HIPCHK(s, hipMemcpy2DAsync(g[i].data(), (size_t)d.cols * 8, s->adj.g[i], (size_t)d.pitch * 8, (size_t)d.cols * 8,
+                               d.rows, hipMemcpyDeviceToHost, s->stream));
+  }
+  std::vector<double> gz[4];
+  for (int side = 0; side < 4; side++) {
+    if (s->cfg.bc[side] != 'C') continue;
+    const tw::DevField& d = p.f[side < 2 ? TW_VX : TW_VY];
+    gz[side].resize(side < 2 ? d.cols : d.rows);
+    HIPCHK(s, hipMemcpyAsync(gz[side].data(), s->adj.gz[side], gz[side].size() * 8, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  // the impedance z = sqrt(kappa / b) of the normal velocity on a
+  // characteristic side (kappa of the adjacent cell): dz/dkappa = z / (2 kappa),
+  // dz/db = -z / (2 b)
+  const int Mx = s->cfg.mx, My = s->cfg.my;
+  const std::vector<double>& kap = s->hcoef[TW_C_KAPPA];
+  for (int side = 0; side < 4; side++) {
+    if (gz[side].empty()) continue;
+    const bool xs = side < 2, right = side % 2;
+    const int f = xs ? TW_VX : TW_VY;
+    const tw::DevField& d = p.f[f];
+    const std::vector<double>& b = s->hcoef[xs ? TW_C_BINV_VX : TW_C_BINV_VY];
+    std::vector<double>& gb = g[xs ? 1 : 2];
+    for (int j = 0; j < (int)gz[side].size(); j++) {
+      const int a = xs ? (right ? d.rows - 1 : 0) : (right ? d.cols - 1 : 0);
+      const size_t qv = xs ? (size_t)a * d.cols + j : (size_t)j * d.cols + a;
+      const int am = xs ? (right ? Mx - 1 : 0) : (right ? My - 1 : 0);
+      const size_t qm = xs ? (size_t)am * My + j : (size_t)j * My + am;
+      const double z = std::sqrt(kap[qm] / b[qv]);
+      g[0][qm] += gz[side][j] * z / (2 * kap[qm]);
+      gb[qv] -= gz[side][j] * z / (2 * b[qv]);
+    }
+  }
+  double* outs[3] = {grad_kappa, grad_binv_vx, grad_binv_vy};
+  for (int i = 0; i < 3; i++)
+    if (outs[i]) memcpy(outs[i], g[i].data(), g[i].size() * 8);
+  return TW_OK;
+}

@@ -102,6 +102,29 @@ struct StepCtl {
   Pending pend;
 };
 
+// Discrete adjoint of the acoustic step (device buffers owned by the solver).
+// Transposed tap lists along an axis: for a partner index j, the outputs i
+// whose stencil reads j and the coefficient (arithmetic type, dt/dx included).
+constexpr int TMAX = 8;
+struct TapList {
+  int32_t* n = nullptr;    // [len] entries per partner index
+  int32_t* idx = nullptr;  // [len][TMAX]
+  void* cf = nullptr;      // [len][TMAX] coefficients
+};
+struct AdjPlan {
+  void* mu[NF] = {};        // adjoint of VX, VY, P (arithmetic type, field pitch)
+  double* g[3] = {};        // d misfit / d coefficient: kappa (MM), 1/rho on NM, 1/rho on MN (field pitch)
+  double* gz[4] = {};       // d misfit / d impedance of the normal velocity along each characteristic side
+  TapList gt[2];            // transpose of the N-output derivative of M data (x, y)
+  TapList dt[2];            // transpose of the M-output derivative of N data (x, y)
+  const double* inj = nullptr;  // receiver adjoint sources [nrcv][nt]
+  int32_t nt = 0;
+};
+// forward fields of a saved state (VX, VY, P storage arrays, field pitch)
+struct StateView {
+  void* s[NF] = {};
+};
+
 // launches (tw_kernels.hip)
 void grid_dims(const Plan& p, dim3* grid, dim3* block);
 int energy_parts(const Plan& p);  // partial sums written by launch_stress_energy
@@ -113,5 +136,10 @@ hipError_t launch_uhat(const Plan& p, hipStream_t s);
 hipError_t launch_stress_energy(const Plan& p, hipStream_t s);
 hipError_t launch_finalize(const Plan& p, const Pending& pend, hipStream_t s);
 hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s);
+// one backward step of the acoustic adjoint for forward step `it`: inject the
+// receiver sources of step it, then apply the transposed step using the saved
+// states after (cur) and before (prev) step it, accumulating the gradients
+hipError_t launch_adjoint_step(const Plan& p, const AdjPlan& a, const StateView& cur, const StateView& prev, int it,
+                               int it_rel, hipStream_t s);
 
 }  // namespace tw

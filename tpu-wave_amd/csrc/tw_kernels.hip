@@ -2176,4 +2176,229 @@ hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------ adjoint
+// Discrete adjoint of one acoustic leapfrog step (no compensation), the
+// transpose of the linear map the forward kernels apply:
+//   V: v <- v + b (G p + S(p, v))     (G: N-output derivative of p, S: SAT)
+//   P: p <- p + kappa (Dv v)          (Dv: M-output derivatives of vx, vy)
+// Step = P o V, so Step^T = V^T o P^T.  With mu the adjoint of the state
+// after the step:
+//   K1: g_kappa += mu_p (Dv v')            (v' = velocities after the step)
+//       mu_v    += Dv^T (kappa mu_p)
+//   K2: g_b     += mu_v (G p + S(p, v))    (p, v before the step)
+//       mu_p    += G^T (b mu_v) + S_p^T (b mu_v)
+//   K3: mu_v    += S_v^T (b mu_v)          (characteristic sides: -a z b)
+// Gradients are accumulated in FP64; the adjoint fields are in the
+// arithmetic type.
+
+template <class CT>
+struct TapT {
+  const int32_t* n;
+  const int32_t* idx;
+  const CT* cf;
+};
+
+template <class CT>
+struct AdjArgs {
+  CT* mu[NF];
+  double* g[3];
+  double* gz[4];
+  TapT<CT> gt[2], dt[2];
+  const double* inj;
+  int32_t nt, it_rel;
+  const Receiver* rcv[2];
+  int32_t nrcv[2];
+};
+
+template <class ST, class KT, class CT>
+__device__ __forceinline__ CT kv(const Stat<ST, KT, CT>& A, int k, int ix, int iy) {
+  return kld<KT, CT>(A.k[k].get(ix, iy));
+}
+
+// K1
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_adj1(Args<ST, KT, CT> U, AdjArgs<CT> J) {
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = blockIdx.y * blockDim.y + threadIdx.y;
+  const F<ST>& P = U.f[TW_P];
+  const int64_t pp = P.pitch;
+  if (ix < P.rows && iy < P.cols) {  // g_kappa
+    const CT d = ddx<ST, CT>(U.f[TW_VX], 0, ix, iy, P.rows, U.perx, U.mx, U.t) +
+                 ddy<ST, CT>(U.f[TW_VY], 0, ix, iy, P.cols, U.pery, U.my, U.t);
+    J.g[0][(int64_t)ix * pp + iy] += (double)J.mu[TW_P][(int64_t)ix * pp + iy] * (double)d;
+  }
+  const F<ST>& X = U.f[TW_VX];
+  if (ix < X.rows && iy < X.cols) {  // mu_vx += Dx^T (kappa mu_p), along x
+    CT acc = (CT)0;
+    const TapT<CT>& L = J.dt[0];
+    for (int e = 0; e < L.n[ix]; e++) {
+      const int i = L.idx[ix * TMAX + e];
+      acc = acc + L.cf[ix * TMAX + e] * (kv(U, TW_C_KAPPA, i, iy) * J.mu[TW_P][(int64_t)i * pp + iy]);
+    }
+    J.mu[TW_VX][(int64_t)ix * X.pitch + iy] += acc;
+  }
+  const F<ST>& Y = U.f[TW_VY];
+  if (ix < Y.rows && iy < Y.cols) {  // mu_vy += Dy^T (kappa mu_p), along y
+    CT acc = (CT)0;
+    const TapT<CT>& L = J.dt[1];
+    for (int e = 0; e < L.n[iy]; e++) {
+      const int j = L.idx[iy * TMAX + e];
+      acc = acc + L.cf[iy * TMAX + e] * (kv(U, TW_C_KAPPA, ix, j) * J.mu[TW_P][(int64_t)ix * pp + j]);
+    }
+    J.mu[TW_VY][(int64_t)ix * Y.pitch + iy] += acc;
+  }
+}
+
+// K2
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_adj2(Args<ST, KT, CT> U, AdjArgs<CT> J) {
+  using S = St<ST>;
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = blockIdx.y * blockDim.y + threadIdx.y;
+  const F<ST>& X = U.f[TW_VX];
+  const F<ST>& Y = U.f[TW_VY];
+  const F<ST>& P = U.f[TW_P];
+  if (ix < X.rows && iy < X.cols) {  // g_bx (and the impedance along characteristic x sides)
+    const int64_t q = X.at(ix, iy);
+    const CT u = (CT)S::ld(X.s[q]);
+    CT t = ddx<ST, CT>(P, 1, ix, iy, X.rows, U.perx, U.mx, U.t);
+    if (!U.perx) t = sat_add(U, t, TW_VX, TW_P, 0, 1, ix, iy, u, (CT)1);
+    const double m = (double)J.mu[TW_VX][q];
+    J.g[1][q] += m * (double)t;
+    for (int side = 0; side < 2; side++)  // increment -a z u b: d/dz = -a u b
+      if (!U.perx && U.bc[side] == 2 && ix == (side == 0 ? 0 : X.rows - 1))
+        J.gz[side][iy] += m * (double)(-((side == 0 ? U.t.ainvN_L : U.t.ainvN_R) * u) * kv(U, TW_C_BINV_VX, ix, iy));
+  }
+  if (ix < Y.rows && iy < Y.cols) {  // g_by (and the impedance along characteristic y sides)
+    const int64_t q = Y.at(ix, iy);
+    const CT u = (CT)S::ld(Y.s[q]);
+    CT t = ddy<ST, CT>(P, 1, ix, iy, Y.cols, U.pery, U.my, U.t);
+    if (!U.pery) t = sat_add(U, t, TW_VY, TW_P, 1, 1, ix, iy, u, (CT)1);
+    const double m = (double)J.mu[TW_VY][q];
+    J.g[2][q] += m * (double)t;
+    for (int side = 0; side < 2; side++)
+      if (!U.pery && U.bc[2 + side] == 2 && iy == (side == 0 ? 0 : Y.cols - 1))
+        J.gz[2 + side][ix] += m * (double)(-((side == 0 ? U.t.ainvN_L : U.t.ainvN_R) * u) * kv(U, TW_C_BINV_VY, ix, iy));
+  }
+  if (ix < P.rows && iy < P.cols) {  // mu_p += G^T (b mu_v) + SAT^T
+    CT acc = (CT)0;
+    const TapT<CT>& Lx = J.gt[0];
+    for (int e = 0; e < Lx.n[ix]; e++) {
+      const int j = Lx.idx[ix * TMAX + e];
+      acc = acc + Lx.cf[ix * TMAX + e] * (kv(U, TW_C_BINV_VX, j, iy) * J.mu[TW_VX][X.at(j, iy)]);
+    }
+    const TapT<CT>& Ly = J.gt[1];
+    for (int e = 0; e < Ly.n[iy]; e++) {
+      const int j = Ly.idx[iy * TMAX + e];
+      acc = acc + Ly.cf[iy * TMAX + e] * (kv(U, TW_C_BINV_VY, ix, j) * J.mu[TW_VY][Y.at(ix, j)]);
+    }
+    // SAT: the boundary velocity point reads the 3-point extrapolation of p
+    if (!U.perx) {
+      if (U.bc[0] && ix < 3)
+        acc = acc + (U.t.ainvN_L * U.t.projM_L[ix]) * (kv(U, TW_C_BINV_VX, 0, iy) * J.mu[TW_VX][X.at(0, iy)]);
+      const int q = ix - (P.rows - 3);
+      if (U.bc[1] && q >= 0)
+        acc = acc - (U.t.ainvN_R * U.t.projM_R[q]) *
+                        (kv(U, TW_C_BINV_VX, X.rows - 1, iy) * J.mu[TW_VX][X.at(X.rows - 1, iy)]);
+    }
+    if (!U.pery) {
+      if (U.bc[2] && iy < 3)
+        acc = acc + (U.t.ainvN_L * U.t.projM_L[iy]) * (kv(U, TW_C_BINV_VY, ix, 0) * J.mu[TW_VY][Y.at(ix, 0)]);
+      const int q = iy - (P.cols - 3);
+      if (U.bc[3] && q >= 0)
+        acc = acc - (U.t.ainvN_R * U.t.projM_R[q]) *
+                        (kv(U, TW_C_BINV_VY, ix, Y.cols - 1) * J.mu[TW_VY][Y.at(ix, Y.cols - 1)]);
+    }
+    J.mu[TW_P][P.at(ix, iy)] += acc;
+  }
+}
+
+// K3: characteristic self-terms on the boundary velocity points
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_adj3(Args<ST, KT, CT> U, AdjArgs<CT> J) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const F<ST>& X = U.f[TW_VX];
+  const F<ST>& Y = U.f[TW_VY];
+  // x sides of vx: rows 0 and mx, columns t
+  if (!U.perx && t < X.cols) {
+    for (int side = 0; side < 2; side++) {
+      if (U.bc[side] != 2) continue;
+      const int a = side == 0 ? 0 : X.rows - 1;
+      const CT ai = side == 0 ? U.t.ainvN_L : U.t.ainvN_R;
+      const CT c = -(ai * U.z[0][side][t]) * kv(U, TW_C_BINV_VX, a, t);
+      CT& m = J.mu[TW_VX][X.at(a, t)];
+      m = m + c * m;
+    }
+  }
+  if (!U.pery && t < Y.rows) {
+    for (int side = 0; side < 2; side++) {
+      if (U.bc[2 + side] != 2) continue;
+      const int a = side == 0 ? 0 : Y.cols - 1;
+      const CT ai = side == 0 ? U.t.ainvN_L : U.t.ainvN_R;
+      const CT c = -(ai * U.z[1][2 + side][t]) * kv(U, TW_C_BINV_VY, t, a);
+      CT& m = J.mu[TW_VY][Y.at(t, a)];
+      m = m + c * m;
+    }
+  }
+}
+
+// receiver sources of forward step it_rel (relative to the adjoint run); one
+// thread, since receivers may share a point
+template <class CT>
+__global__ void k_adj_inject(AdjArgs<CT> J, int64_t p_vx, int64_t p_vy, int64_t p_p) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int g = 0; g < 2; g++)
+    for (int r = 0; r < J.nrcv[g]; r++) {
+      const Receiver rc = J.rcv[g][r];
+      const int64_t pitch = rc.field == TW_VX ? p_vx : (rc.field == TW_VY ? p_vy : p_p);
+      CT& m = J.mu[rc.field][(int64_t)rc.ix * pitch + rc.iy];
+      m = m + (CT)J.inj[(int64_t)rc.row * J.nt + J.it_rel];
+    }
+}
+
+template <class ST, class KT, class CT>
+static hipError_t run_adjoint_step(const Plan& p, const AdjPlan& a, const StateView& cur, const StateView& prev,
+                                   int it, int it_rel, hipStream_t s) {
+  AdjArgs<CT> J;
+  for (int f = 0; f < NF; f++) J.mu[f] = (CT*)a.mu[f];
+  for (int i = 0; i < 3; i++) J.g[i] = a.g[i];
+  for (int i = 0; i < 4; i++) J.gz[i] = a.gz[i];
+  for (int d = 0; d < 2; d++) {
+    J.gt[d] = TapT<CT>{a.gt[d].n, a.gt[d].idx, (const CT*)a.gt[d].cf};
+    J.dt[d] = TapT<CT>{a.dt[d].n, a.dt[d].idx, (const CT*)a.dt[d].cf};
+  }
+  J.inj = a.inj;
+  J.nt = a.nt;
+  J.it_rel = it_rel;
+  J.rcv[0] = p.rcv_v;
+  J.rcv[1] = p.rcv_s;
+  J.nrcv[0] = p.nrcv_v;
+  J.nrcv[1] = p.nrcv_s;
+  StepCtl c;
+  c.it = it;
+  auto U1 = make_args<ST, KT, CT>(p, &c, nullptr);
+  auto U0 = U1;
+  for (int f = 0; f < NF; f++) {
+    if (cur.s[f]) U1.f[f].s = (ST*)cur.s[f];
+    if (prev.s[f]) U0.f[f].s = (ST*)prev.s[f];
+  }
+  hipLaunchKernelGGL((k_adj_inject<CT>), dim3(1), dim3(64), 0, s, J, p.f[TW_VX].pitch, p.f[TW_VY].pitch,
+                     p.f[TW_P].pitch);
+  const dim3 b(64, 4), g((unsigned)((p.my + 1 + 63) / 64), (unsigned)((p.mx + 1 + 3) / 4));
+  hipLaunchKernelGGL((k_adj1<ST, KT, CT>), g, b, 0, s, U1, J);
+  hipLaunchKernelGGL((k_adj2<ST, KT, CT>), g, b, 0, s, U0, J);
+  const int n = (p.mx > p.my ? p.mx : p.my) + 1;
+  hipLaunchKernelGGL((k_adj3<ST, KT, CT>), dim3((n + 255) / 256), dim3(256), 0, s, U0, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_step(const Plan& p, const AdjPlan& a, const StateView& cur, const StateView& prev, int it,
+                               int it_rel, hipStream_t s) {
+  switch (p.precision) {
+    case TW_FP64: return run_adjoint_step<double, double, double>(p, a, cur, prev, it, it_rel, s);
+    case TW_FP32: return run_adjoint_step<float, float, float>(p, a, cur, prev, it, it_rel, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace tw

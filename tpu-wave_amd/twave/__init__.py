@@ -101,6 +101,8 @@ _SIGS = [
     ("tw_checkpoint_load", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     ("tw_set_stream", ctypes.c_int, [_vp, _vp]),
     ("tw_check_finite", ctypes.c_int, [_vp]),
+    ("tw_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    ("tw_adjoint_run", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
 ]
 
 
@@ -315,6 +317,22 @@ class Solver:
 
     def set_stream(self, stream_ptr):
         self._chk(lib().tw_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
+
+    def adjoint_forward(self, nsteps, interval=0):
+        """Advance nsteps keeping sqrt(N) checkpoints for adjoint_run
+        (acoustic, FP64/FP32 storage, cpst 0)."""
+        self._chk(lib().tw_adjoint_forward(self.h, int(nsteps), int(interval)))
+
+    def adjoint_run(self, rcv_adjoint):
+        """dL/d(coefficients) for rcv_adjoint[n_rcv][nsteps] = dL/d(trace) of
+        the last adjoint_forward run.  Returns (d/d kappa on MM, d/d(1/rho) on
+        NM, d/d(1/rho) on MN)."""
+        w = np.ascontiguousarray(rcv_adjoint, dtype=np.float64)
+        gk = np.zeros(self.get_coef(C_KAPPA).shape)
+        gx = np.zeros(self.get_coef(C_BINV_VX).shape)
+        gy = np.zeros(self.get_coef(C_BINV_VY).shape)
+        self._chk(lib().tw_adjoint_run(self.h, _d(w), _d(gk), _d(gx), _d(gy)))
+        return gk, gx, gy
 
     def check_finite(self):
         e = lib().tw_check_finite(self.h)
