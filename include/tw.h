@@ -236,6 +236,30 @@ int tw_adjoint_forward(tw_solver* s, int32_t nsteps, int32_t interval);
 int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, double* grad_binv_vx,
                    double* grad_binv_vy);
 
+/* ---- batched shots ----
+ *
+ * No counterpart in the reference, which runs its shots one after another
+ * (cuda/src/cpp/main_host_FDM_2D.tpp:21-221 loops over sources and calls the
+ * forward simulation for each).  A batch groups independent solvers (same
+ * physics, storage, compensation and device; any sizes, materials, sources
+ * and boundaries) and advances them together: one launch per half step for
+ * the whole batch, the time loop captured in a hipGraph and replayed when the
+ * same step range is run again.  Batched steps record no energy and no
+ * receiver samples.  The solvers stay usable on their own; calls on them are
+ * ordered after the batch's work. */
+typedef struct tw_batch tw_batch;
+int tw_batch_create(tw_solver* const* solvers, int32_t n, tw_batch** out);
+int tw_batch_destroy(tw_batch* b);
+const char* tw_batch_last_error(const tw_batch* b);
+/* Advance every solver of the batch by nsteps. */
+int tw_batch_step(tw_batch* b, int32_t nsteps);
+/* Rows per block of the core launches (the tile height). */
+int tw_batch_set_rows(tw_batch* b, int32_t rows);
+/* Time trial_steps batched steps for each candidate tile height (4..64 rows),
+ * keep the fastest, and leave the solvers' states and step counters as they
+ * were. */
+int tw_batch_autotune(tw_batch* b, int32_t trial_steps, int32_t* best_rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1288,3 +1288,284 @@ int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, 
     if (outs[i]) memcpy(outs[i], g[i].data(), g[i].size() * 8);
   return TW_OK;
 }
+
+// ------------------------------------------------------------ batched shots
+struct tw_batch {
+  std::vector<tw_solver*> sv;
+  int device = 0;
+  int rows = 16;  // chunk height (rows per block) of the core launches
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  // cached graph of the last step range
+  hipGraphExec_t exec = nullptr;
+  int g_nsteps = -1, g_rows = -1;
+  std::vector<int> g_it0;
+  void* items = nullptr;
+  size_t items_cap = 0;
+  // per step and half: item offsets and launch sizes
+  struct Half {
+    size_t core_off, band_off;
+    int ncore, core_blocks, nband, band_blocks;
+  };
+  std::vector<Half> halves;
+  std::string err;
+};
+
+namespace {
+
+int bfail(tw_batch* b, int code, const std::string& m) { return fail(b ? &b->err : nullptr, code, m); }
+
+#define BHIP(b, call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return bfail((b), TW_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+tw::Dyn batch_dyn(tw_solver* s, int it, int half) {
+  tw::Dyn d{};
+  d.it = it;
+  d.energy = 0;
+  const bool vsrc = s->src_grid == TW_GRID_MN || s->src_grid == TW_GRID_NM;
+  if (s->src_grid >= 0 && (half == 0) == vsrc && it < (int)s->src_inc.size()) {
+    d.src_on = 1;
+    d.src_grid = s->src_grid;
+    d.src_ix = s->src_ix;
+    d.src_iy = s->src_iy;
+    d.src_inc = s->src_inc[it];
+  } else {
+    d.src_grid = -1;
+  }
+  d.part = half == 0 ? s->plan.part_v : s->plan.part_s;
+  d.rec = s->plan.rec;
+  d.nt_max = s->plan.nt_max;
+  return d;
+}
+
+// item tables of nsteps steps from each solver's current step, in device
+// memory, and the graph that launches them
+int batch_build(tw_batch* b, int nsteps) {
+  const int n = (int)b->sv.size();
+  std::vector<tw::BatchCore> core;
+  std::vector<tw::BatchBand> band;
+  b->halves.clear();
+  for (int t = 0; t < nsteps; t++)
+    for (int half = 0; half < 2; half++) {
+      tw_batch::Half H{core.size(), band.size(), 0, 0, 0, 0};
+      for (int k = 0; k < n; k++) {
+        tw_solver* s = b->sv[k];
+        s->plan.txc = b->rows;
+        tw::BatchCore c{};
+        tw::BatchBand r{};
+        int nc, nb;
+        tw::batch_geometry(s->plan, &c, &nc, &r, &nb);
+        c.sp = r.sp = s->plan.dstat;
+        c.d = r.d = batch_dyn(s, s->it + t, half);
+        if (nc > 0) {
+          c.b0 = H.core_blocks;
+          H.core_blocks += nc;
+          core.push_back(c);
+          H.ncore++;
+        }
+        if (nb > 0) {
+          r.b0 = H.band_blocks;
+          H.band_blocks += nb;
+          band.push_back(r);
+          H.nband++;
+        }
+      }
+      b->halves.push_back(H);
+    }
+  const size_t cb = core.size() * sizeof(tw::BatchCore), bb = band.size() * sizeof(tw::BatchBand);
+  if (b->items_cap < cb + bb) {
+    if (b->items) (void)hipFree(b->items);
+    b->items = nullptr;
+    BHIP(b, hipMalloc(&b->items, cb + bb));
+    b->items_cap = cb + bb;
+  }
+  BHIP(b, hipMemcpyAsync(b->items, core.data(), cb, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipMemcpyAsync((char*)b->items + cb, band.data(), bb, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  const tw::BatchCore* dc = (const tw::BatchCore*)b->items;
+  const tw::BatchBand* db = (const tw::BatchBand*)((char*)b->items + cb);
+  const Plan& p0 = b->sv[0]->plan;
+  hipGraph_t g = nullptr;
+  BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < (int)b->halves.size(); i++) {
+    const tw_batch::Half& H = b->halves[i];
+    hipError_t e = tw::launch_batch_half(p0, i % 2, dc + H.core_off, H.ncore, H.core_blocks, db + H.band_off, H.nband,
+                                         H.band_blocks, b->stream);
+    if (e != hipSuccess) {
+      (void)hipStreamEndCapture(b->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return bfail(b, TW_EDEVICE, std::string("batched launch: ") + hipGetErrorString(e));
+    }
+  }
+  BHIP(b, hipStreamEndCapture(b->stream, &g));
+  hipError_t e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return bfail(b, TW_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  b->g_nsteps = nsteps;
+  b->g_rows = b->rows;
+  b->g_it0.clear();
+  for (tw_solver* s : b->sv) b->g_it0.push_back(s->it);
+  return TW_OK;
+}
+
+bool batch_cached(const tw_batch* b, int nsteps) {
+  if (!b->exec || b->g_nsteps != nsteps || b->g_rows != b->rows) return false;
+  for (size_t k = 0; k < b->sv.size(); k++)
+    if (b->g_it0[k] != b->sv[k]->it) return false;
+  return true;
+}
+
+// the graph launch on the batch stream, ordered after each solver's queued
+// work and before anything queued on the solvers afterwards
+int batch_launch(tw_batch* b, int nsteps, hipEvent_t t0, hipEvent_t t1) {
+  for (tw_solver* s : b->sv) {
+    BHIP(b, hipEventRecord(b->done, s->stream));
+    BHIP(b, hipStreamWaitEvent(b->stream, b->done, 0));
+  }
+  if (t0) BHIP(b, hipEventRecord(t0, b->stream));
+  BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  if (t1) BHIP(b, hipEventRecord(t1, b->stream));
+  BHIP(b, hipEventRecord(b->done, b->stream));
+  for (tw_solver* s : b->sv) {
+    BHIP(b, hipStreamWaitEvent(s->stream, b->done, 0));
+    s->it += nsteps;
+    s->ep_valid = false;
+  }
+  return TW_OK;
+}
+
+// device copies of every field array (state and companions) of the batch
+int batch_state(tw_batch* b, std::vector<void*>& bufs, bool save) {
+  size_t k = 0;
+  for (tw_solver* s : b->sv) {
+    const size_t es = state_size(s->cfg.precision);
+    for (int f = 0; f < tw::NF; f++) {
+      const tw::DevField& d = s->plan.f[f];
+      void* arr[3] = {d.s, d.r, d.c};
+      for (void* a : arr) {
+        if (!a) continue;
+        const size_t bytes = (size_t)d.rows * d.pitch * es;
+        if (save) {
+          if (k == bufs.size()) {
+            void* q = nullptr;
+            BHIP(b, hipMalloc(&q, bytes));
+            bufs.push_back(q);
+          }
+          BHIP(b, hipMemcpyAsync(bufs[k], a, bytes, hipMemcpyDeviceToDevice, b->stream));
+        } else {
+          BHIP(b, hipMemcpyAsync(a, bufs[k], bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
+        k++;
+      }
+    }
+  }
+  return TW_OK;
+}
+
+}  // namespace
+
+int tw_batch_create(tw_solver* const* solvers, int32_t n, tw_batch** out) {
+  if (!solvers || n <= 0 || !out) return fail(nullptr, TW_EINVAL, "bad argument");
+  *out = nullptr;
+  const tw_config& c0 = solvers[0]->cfg;
+  for (int k = 0; k < n; k++) {
+    const tw_solver* s = solvers[k];
+    if (!s) return fail(nullptr, TW_EINVAL, "null solver");
+    const tw_config& c = s->cfg;
+    if (c.physics != c0.physics || c.precision != c0.precision || c.coef16 != c0.coef16 || c.cpst != c0.cpst ||
+        c.device != c0.device || c.energy_form != c0.energy_form)
+      return fail(nullptr, TW_EINVAL, "batch: solvers must share physics, precision, coef16, cpst, device, energy form");
+    if (s->need_uhat) return fail(nullptr, TW_EINVAL, "batch: characteristic sides of elastic problems are not batched");
+    if (s->plan.kver == 1 || c.cpst == TW_CPST_3C || c.cpst == TW_CPST_M3C || c.cpst == TW_CPST_6C)
+      return fail(nullptr, TW_EINVAL, "batch: streaming kernels only (no 3C/-3C/6C)");
+  }
+  (void)hipSetDevice(c0.device);
+  tw_batch* b = new tw_batch();
+  b->sv.assign(solvers, solvers + n);
+  b->device = c0.device;
+  b->rows = solvers[0]->plan.txc;
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    tw_batch_destroy(b);
+    return fail(nullptr, TW_EDEVICE, "batch: stream or event creation failed");
+  }
+  *out = b;
+  return TW_OK;
+}
+
+int tw_batch_destroy(tw_batch* b) {
+  if (!b) return TW_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->items) (void)hipFree(b->items);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return TW_OK;
+}
+
+const char* tw_batch_last_error(const tw_batch* b) { return b ? b->err.c_str() : g_err.c_str(); }
+
+int tw_batch_set_rows(tw_batch* b, int32_t rows) {
+  if (!b || rows <= 0) return bfail(b, TW_EINVAL, "bad argument");
+  b->rows = rows;
+  return TW_OK;
+}
+
+int tw_batch_step(tw_batch* b, int32_t nsteps) {
+  if (!b || nsteps < 0) return bfail(b, TW_EINVAL, "bad argument");
+  if (nsteps == 0) return TW_OK;
+  (void)hipSetDevice(b->device);
+  int e;
+  if (!batch_cached(b, nsteps) && (e = batch_build(b, nsteps))) return e;
+  return batch_launch(b, nsteps, nullptr, nullptr);
+}
+
+int tw_batch_autotune(tw_batch* b, int32_t trial_steps, int32_t* best_rows) {
+  if (!b || trial_steps <= 0) return bfail(b, TW_EINVAL, "bad argument");
+  (void)hipSetDevice(b->device);
+  std::vector<void*> bufs;
+  std::vector<int> it0;
+  for (tw_solver* s : b->sv) {
+    BHIP(b, hipStreamSynchronize(s->stream));
+    it0.push_back(s->it);
+  }
+  int e = batch_state(b, bufs, true);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  BHIP(b, hipEventCreate(&t0));
+  BHIP(b, hipEventCreate(&t1));
+  const int cand[] = {4, 8, 16, 32, 64};
+  float best = 0.f;
+  int best_r = -1;
+  for (int r : cand) {
+    if (e) break;
+    b->rows = r;
+    if ((e = batch_build(b, trial_steps))) break;
+    if ((e = batch_launch(b, trial_steps, t0, t1))) break;
+    float ms = 0.f;
+    if (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+      e = bfail(b, TW_EDEVICE, "batch autotune timing");
+      break;
+    }
+    if (best_r < 0 || ms < best) {
+      best = ms;
+      best_r = r;
+    }
+    if ((e = batch_state(b, bufs, false))) break;
+    for (size_t k = 0; k < b->sv.size(); k++) b->sv[k]->it = it0[k];
+  }
+  (void)hipStreamSynchronize(b->stream);
+  for (void* q : bufs) (void)hipFree(q);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (e) return e;
+  b->rows = best_r;
+  for (tw_solver* s : b->sv) s->plan.txc = best_r;
+  if (best_rows) *best_rows = best_r;
+  return TW_OK;
+}

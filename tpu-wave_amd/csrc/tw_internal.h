@@ -125,6 +125,57 @@ struct StateView {
   void* s[NF] = {};
 };
 
+// Per-launch control.
+struct Dyn {
+  int32_t it, energy;
+  int32_t src_on, src_grid, src_ix, src_iy;
+  double src_inc;
+  double* part;
+  Pending pend;
+  double* rec;
+  int32_t nt_max;
+};
+
+// Rectangles of the union index space covered by a per-point launch.  n = 0:
+// the whole (mx+1) x (my+1) space on a 2-D grid.  Otherwise blockIdx.x runs
+// over the tiles of the n rectangles in order (the SBP boundary band around
+// the core region of the streaming kernels).  A tile is 256 points: tw[k]
+// columns by 256 / tw[k] rows, tw = 4 or 8 for the narrow column bands so no
+// thread of a block idles, 64 otherwise.
+struct Rects {
+  int32_t n;
+  int32_t r0[4], r1[4], c0[4], c1[4];
+  int32_t tw[4];   // tile width (columns) of each rectangle
+  int32_t cum[5];  // first tile of each rectangle
+  int32_t poff;    // offset of this launch's energy partials
+};
+__host__ __device__ inline int tile_width(int cols) { return cols <= 4 ? 4 : (cols <= 8 ? 8 : 64); }
+
+// Core region of a half step: rows [rlo, rhi) and columns [clo, chi) of the
+// union index space, away from SBP closures (all rows / columns along a
+// periodic axis).  The boundary band around it is done by the per-point
+// kernels launched over the band rectangles.
+struct Core {
+  int32_t rlo, rhi, clo, chi, txc;
+};
+
+// Batched shots: one problem's share of a batched half-step launch.
+struct BatchCore {
+  const void* sp;  // the problem's device Stat
+  Dyn d;
+  Core c;
+  int32_t b0;      // first block of this problem in the launch
+  int32_t nbx;     // strips
+  int32_t nb;      // blocks
+};
+struct BatchBand {
+  const void* sp;
+  Dyn d;
+  Rects r;
+  int32_t b0;
+  int32_t nb;
+};
+
 // launches (tw_kernels.hip)
 void grid_dims(const Plan& p, dim3* grid, dim3* block);
 int energy_parts(const Plan& p);  // partial sums written by launch_stress_energy
@@ -139,6 +190,10 @@ hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s);
 // one backward step of the acoustic adjoint for forward step `it`: inject the
 // receiver sources of step it, then apply the transposed step using the saved
 // states after (cur) and before (prev) step it, accumulating the gradients
+hipError_t launch_batch_half(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
+                             const BatchBand* band, int nband, int band_blocks, hipStream_t s);
+// core and band share of one problem (d, sp and b0 are left to the caller)
+void batch_geometry(const Plan& p, BatchCore* core, int* core_blocks, BatchBand* band, int* band_blocks);
 hipError_t launch_adjoint_step(const Plan& p, const AdjPlan& a, const StateView& cur, const StateView& prev, int it,
                                int it_rel, hipStream_t s);
 

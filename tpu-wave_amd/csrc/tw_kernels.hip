@@ -117,16 +117,6 @@ struct Stat {
   CT* uhat[2][4];
 };
 
-// Per-launch control.
-struct Dyn {
-  int32_t it, energy;
-  int32_t src_on, src_grid, src_ix, src_iy;
-  double src_inc;
-  double* part;
-  Pending pend;
-  double* rec;
-  int32_t nt_max;
-};
 
 template <class ST, class KT, class CT>
 struct Args : Stat<ST, KT, CT>, Dyn {};
@@ -505,30 +495,16 @@ __device__ void do_pending(const Stat<ST, KT, CT>& A, const Dyn& D, double* sh) 
 }
 
 // ------------------------------------------------------------ half steps
-// Rectangles of the union index space covered by a per-point launch.  n = 0:
-// the whole (mx+1) x (my+1) space on a 2-D grid.  Otherwise blockIdx.x runs
-// over the tiles of the n rectangles in order (the SBP boundary band around
-// the core region of the streaming kernels).  A tile is 256 points: tw[k]
-// columns by 256 / tw[k] rows, tw = 4 or 8 for the narrow column bands so no
-// thread of a block idles, 64 otherwise.
-struct Rects {
-  int32_t n;
-  int32_t r0[4], r1[4], c0[4], c1[4];
-  int32_t tw[4];   // tile width (columns) of each rectangle
-  int32_t cum[5];  // first tile of each rectangle
-  int32_t poff;    // offset of this launch's energy partials
-};
 
-__host__ __device__ inline int tile_width(int cols) { return cols <= 4 ? 4 : (cols <= 8 ? 8 : 64); }
 
-__device__ __forceinline__ bool map_point(const Rects& R, int* ix, int* iy, int* pidx) {
+__device__ __forceinline__ bool map_point(const Rects& R, int bx, int by, int* ix, int* iy, int* pidx) {
   if (R.n == 0) {
-    *iy = blockIdx.x * blockDim.x + threadIdx.x;
-    *ix = blockIdx.y * blockDim.y + threadIdx.y;
-    *pidx = blockIdx.y * gridDim.x + blockIdx.x;
+    *iy = bx * blockDim.x + threadIdx.x;
+    *ix = by * blockDim.y + threadIdx.y;
+    *pidx = by * gridDim.x + bx;
     return true;
   }
-  const int b = blockIdx.x;
+  const int b = bx;
   int k = 0;
   while (k + 1 < R.n && b >= R.cum[k + 1]) k++;
   const int t = b - R.cum[k];
@@ -543,11 +519,11 @@ __device__ __forceinline__ bool map_point(const Rects& R, int* ix, int* iy, int*
 
 // Velocity half step: vx (NM) and vy (MN).
 template <class ST, class KT, class CT>
-__global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
-  __shared__ double sh[8];
-  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, A, sh);
+__device__ __forceinline__ void vel_pt(const Stat<ST, KT, CT>& A, const Dyn& D, const Rects& R, int bx, int by,
+                                       double* sh) {
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   int ix, iy, pidx;
-  const bool inr = map_point(R, &ix, &iy, &pidx);
+  const bool inr = map_point(R, bx, by, &ix, &iy, &pidx);
   const int el = A.physics == TW_ELASTIC;
   const int kmode = A.cpst == TW_CPST_K;
   double e = 0.0;
@@ -571,11 +547,11 @@ __global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
       ST sn = X.s[q], rn = sn;
       if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) rn = X.r[q];
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
-      if (A.src_on && A.src_grid == TW_GRID_NM && ix == A.src_ix && iy == A.src_iy)
-        add_source<ST, CT>(&sn, &rn, A.src_inc, kmode);
+      if (D.src_on && D.src_grid == TW_GRID_NM && ix == D.src_ix && iy == D.src_iy)
+        add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
       X.s[q] = sn;
       if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.r[q] = rn;
-      if (A.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 1, 0, ix, iy, X.rows, X.cols);
+      if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 1, 0, ix, iy, X.rows, X.cols);
     }
   }
   // ---- vy on MN: d/dx of sxy (elastic) + d/dy of syy (elastic) or p
@@ -598,26 +574,32 @@ __global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
       ST sn = X.s[q], rn = sn;
       if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) rn = X.r[q];
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
-      if (A.src_on && A.src_grid == TW_GRID_MN && ix == A.src_ix && iy == A.src_iy)
-        add_source<ST, CT>(&sn, &rn, A.src_inc, kmode);
+      if (D.src_on && D.src_grid == TW_GRID_MN && ix == D.src_ix && iy == D.src_iy)
+        add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
       X.s[q] = sn;
       if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.r[q] = rn;
-      if (A.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 0, 1, ix, iy, X.rows, X.cols);
+      if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 0, 1, ix, iy, X.rows, X.cols);
     }
   }
-  if (A.energy) {
+  if (D.energy) {
     const double bsum = block_sum(e, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) A.part[pidx] = bsum;
+    if (threadIdx.x == 0 && threadIdx.y == 0) D.part[pidx] = bsum;
   }
+}
+
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
+  __shared__ double sh[8];
+  vel_pt<ST, KT, CT>(A, A, R, blockIdx.x, blockIdx.y, sh);
 }
 
 // Stress half step: sxy (NN) and sxx, syy (MM), or p (MM, acoustic).
 template <class ST, class KT, class CT>
-__global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
-  __shared__ double sh[8];
-  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, A, sh);
+__device__ __forceinline__ void str_pt(const Stat<ST, KT, CT>& A, const Dyn& D, const Rects& R, int bx, int by,
+                                       double* sh) {
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   int ix, iy, pidx;
-  const bool inr = map_point(R, &ix, &iy, &pidx);
+  const bool inr = map_point(R, bx, by, &ix, &iy, &pidx);
   const int kmode = A.cpst == TW_CPST_K;
   const int rmode = kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R;
   double e = 0.0;
@@ -635,11 +617,11 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
         ST sn = X.s[q], rn = sn;
         if (rmode) rn = X.r[q];
         update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
-        if (A.src_on && A.src_grid == TW_GRID_NN && ix == A.src_ix && iy == A.src_iy)
-          add_source<ST, CT>(&sn, &rn, A.src_inc, kmode);
+        if (D.src_on && D.src_grid == TW_GRID_NN && ix == D.src_ix && iy == D.src_iy)
+          add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
         X.s[q] = sn;
         if (rmode) X.r[q] = rn;
-        if (A.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), mu, 1, 1, ix, iy, X.rows, X.cols);
+        if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), mu, 1, 1, ix, iy, X.rows, X.cols);
       }
     }
     {
@@ -666,9 +648,9 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
         }
         update<ST, CT>(X0, q, a0, A.cpst, &s0, &r0);
         update<ST, CT>(X1, q, a1, A.cpst, &s1, &r1);
-        if (A.src_on && A.src_grid == TW_GRID_MM && ix == A.src_ix && iy == A.src_iy) {
-          add_source<ST, CT>(&s0, &r0, A.src_inc, kmode);
-          add_source<ST, CT>(&s1, &r1, A.src_inc, kmode);
+        if (D.src_on && D.src_grid == TW_GRID_MM && ix == D.src_ix && iy == D.src_iy) {
+          add_source<ST, CT>(&s0, &r0, D.src_inc, kmode);
+          add_source<ST, CT>(&s1, &r1, D.src_inc, kmode);
         }
         X0.s[q] = s0;
         X1.s[q] = s1;
@@ -676,7 +658,7 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
           X0.r[q] = r0;
           X1.r[q] = r1;
         }
-        if (A.energy) {
+        if (D.energy) {
           const double L = (double)lam, M2 = (double)tmu;
           double p1 = (L + M2) / (M2 * (2. * L + M2));
           double p2 = -(L * 2.) / (M2 * (2. * L + M2));
@@ -702,21 +684,27 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
       ST sn = X.s[q], rn = sn;
       if (rmode) rn = X.r[q];
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
-      if (A.src_on && A.src_grid == TW_GRID_MM && ix == A.src_ix && iy == A.src_iy)
-        add_source<ST, CT>(&sn, &rn, A.src_inc, kmode);
+      if (D.src_on && D.src_grid == TW_GRID_MM && ix == D.src_ix && iy == D.src_iy)
+        add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
       X.s[q] = sn;
       if (rmode) X.r[q] = rn;
-      if (A.energy) {
+      if (D.energy) {
         const CT v = value_of<ST, CT>(sn, rn, kmode);
         if (A.eform == 1) e += energy_pt(A, v, kap, 0, 0, ix, iy, X.rows, X.cols);
         else e += energy_pt(A, v, kap, 0, 0, ix, iy, X.rows, X.cols);
       }
     }
   }
-  if (A.energy) {
+  if (D.energy) {
     const double bsum = block_sum(e, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) A.part[pidx] = bsum;
+    if (threadIdx.x == 0 && threadIdx.y == 0) D.part[pidx] = bsum;
   }
+}
+
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
+  __shared__ double sh[8];
+  str_pt<ST, KT, CT>(A, A, R, blockIdx.x, blockIdx.y, sh);
 }
 
 // Energy of the stress fields only (start of a run / after the state was set).
@@ -967,13 +955,6 @@ __device__ __forceinline__ void shift_win(T w[4][4]) {
     for (int j = 0; j < 4; j++) w[k][j] = w[k + 1][j];
 }
 
-// Core region of a half step: rows [rlo, rhi) and columns [clo, chi) of the
-// union index space, away from SBP closures (all rows / columns along a
-// periodic axis).  The boundary band around it is done by the per-point
-// kernels launched over the band rectangles.
-struct Core {
-  int32_t rlo, rhi, clo, chi, txc;
-};
 
 // field parameters held in registers for the whole kernel
 // global (address space 1) pointer: the field pointers come from a struct in
@@ -1338,13 +1319,14 @@ __device__ __forceinline__ double e_mm(CT lam, CT tmu, CT sxx, CT syy) {
 
 // Velocity half step, core region, streaming.
 template <class ST, class KT, class CT, int PH, int CP>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>))) k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+__device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
+                                          int by, int nbx) {
   constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // sxy row (y partner of vx, elastic)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // syy or p row (y partner of vy)
   __shared__ double sh[8];
   const Stat<ST, KT, CT>& A = *Sp;
-  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, D, sh);
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   FR<ST> VX, VY, PX, PY, SXY;
   VX.load(A.f[TW_VX]);
   VY.load(A.f[TW_VY]);
@@ -1363,9 +1345,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST,
   const int energy = D.energy;
 
   const int lane = threadIdx.x;
-  const int base = blockIdx.x * STRIP;
+  const int base = bx * STRIP;
   const int c = base + lane * VW;
-  const int x0 = C.rlo + blockIdx.y * C.txc;
+  const int x0 = C.rlo + by * C.txc;
   const int x1 = min(x0 + C.txc, C.rhi);
   // fast path: every strip of an SBP y axis (columns masked to the core,
   // loads clamped inside the allocation), interior strips of a periodic one
@@ -1586,19 +1568,26 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST,
     rows(std::false_type{});
   if (energy) {
     const double bsum = block_sum(e, sh);
-    if (lane == 0) D.part[blockIdx.y * gridDim.x + blockIdx.x] = bsum;
+    if (lane == 0) D.part[by * nbx + bx] = bsum;
   }
+}
+
+template <class ST, class KT, class CT, int PH, int CP>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
+k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+  vel3_body<ST, KT, CT, PH, CP>(Sp, D, C, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // Stress half step, core region, streaming.
 template <class ST, class KT, class CT, int PH, int CP>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>))) k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+__device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
+                                          int by, int nbx) {
   constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // vx row (y partner of sxy)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // vy row (y partner of MM / p)
   __shared__ double sh[8];
   const Stat<ST, KT, CT>& A = *Sp;
-  if (blockIdx.x == 0 && blockIdx.y == 0) do_pending<ST, KT, CT>(A, D, sh);
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   FR<ST> VX, VY, XN, X0, X1;
   VX.load(A.f[TW_VX]);
   VY.load(A.f[TW_VY]);
@@ -1621,9 +1610,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST,
   const int energy = D.energy;
 
   const int lane = threadIdx.x;
-  const int base = blockIdx.x * STRIP;
+  const int base = bx * STRIP;
   const int c = base + lane * VW;
-  const int x0 = C.rlo + blockIdx.y * C.txc;
+  const int x0 = C.rlo + by * C.txc;
   const int x1 = min(x0 + C.txc, C.rhi);
   // fast path: every strip of an SBP y axis (columns masked to the core,
   // loads clamped inside the allocation), interior strips of a periodic one
@@ -1929,8 +1918,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST,
     rows(std::false_type{});
   if (energy) {
     const double bsum = block_sum(e, sh);
-    if (lane == 0) D.part[blockIdx.y * gridDim.x + blockIdx.x] = bsum;
+    if (lane == 0) D.part[by * nbx + bx] = bsum;
   }
+}
+
+template <class ST, class KT, class CT, int PH, int CP>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
+k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+  str3_body<ST, KT, CT, PH, CP>(Sp, D, C, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // ------------------------------------------------------------ dispatch
@@ -2174,6 +2169,124 @@ hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s) {
     }
   }
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------ batched shots
+// One launch per half step for a batch of independent problems: the blocks of
+// all problems' core (or band) grids are numbered consecutively, and each
+// block finds its problem by a binary search over the first-block offsets,
+// then runs the same body as the single-problem kernels on that problem's
+// device Stat with the step's Dyn.
+__device__ __forceinline__ int batch_item(const int32_t* b0, int stride_words, int n, int block) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b0[(int64_t)mid * stride_words] <= block)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class ST, class KT, class CT, int PH, int CP>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
+k_vel3b(const BatchCore* __restrict__ items, int n) {
+  const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, blockIdx.x);
+  const BatchCore& B = items[k];
+  const int l = blockIdx.x - B.b0;
+  vel3_body<ST, KT, CT, PH, CP>((const Stat<ST, KT, CT>*)B.sp, B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
+}
+
+template <class ST, class KT, class CT, int PH, int CP>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
+k_str3b(const BatchCore* __restrict__ items, int n) {
+  const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, blockIdx.x);
+  const BatchCore& B = items[k];
+  const int l = blockIdx.x - B.b0;
+  str3_body<ST, KT, CT, PH, CP>((const Stat<ST, KT, CT>*)B.sp, B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
+}
+
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_bandb(const BatchBand* __restrict__ items, int n, int half) {
+  __shared__ double sh[8];
+  const int k = batch_item(&items[0].b0, sizeof(BatchBand) / 4, n, blockIdx.x);
+  const BatchBand& B = items[k];
+  const Stat<ST, KT, CT>& A = *(const Stat<ST, KT, CT>*)B.sp;
+  if (half == 0)
+    vel_pt<ST, KT, CT>(A, B.d, B.r, blockIdx.x - B.b0, 0, sh);
+  else
+    str_pt<ST, KT, CT>(A, B.d, B.r, blockIdx.x - B.b0, 0, sh);
+}
+
+template <class ST, class KT, class CT, int PH>
+static void launch_batch_core(const Plan& p, int half, const BatchCore* items, int n, int blocks, hipStream_t s) {
+  auto go = [&](auto cp) {
+    constexpr int CP = decltype(cp)::value;
+    if (half == 0)
+      hipLaunchKernelGGL((k_vel3b<ST, KT, CT, PH, CP>), dim3(blocks), dim3(64), 0, s, items, n);
+    else
+      hipLaunchKernelGGL((k_str3b<ST, KT, CT, PH, CP>), dim3(blocks), dim3(64), 0, s, items, n);
+  };
+  if constexpr (sizeof(ST) >= 4) {
+    switch (p.cpst) {
+      case TW_CPST_3R: go(std::integral_constant<int, TW_CPST_3R>{}); break;
+      case TW_CPST_M3R: go(std::integral_constant<int, TW_CPST_M3R>{}); break;
+      case TW_CPST_6R: go(std::integral_constant<int, TW_CPST_6R>{}); break;
+      default: go(std::integral_constant<int, TW_CPST_0>{}); break;
+    }
+  } else {
+    if (p.cpst == TW_CPST_K)
+      go(std::integral_constant<int, TW_CPST_K>{});
+    else
+      go(std::integral_constant<int, TW_CPST_0>{});
+  }
+}
+
+template <class ST, class KT, class CT>
+static hipError_t run_batch_half(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
+                                 const BatchBand* band, int nband, int band_blocks, hipStream_t s) {
+  if (ncore > 0) {
+    if (p.physics == TW_ELASTIC)
+      launch_batch_core<ST, KT, CT, TW_ELASTIC>(p, half, core, ncore, core_blocks, s);
+    else
+      launch_batch_core<ST, KT, CT, TW_ACOUSTIC>(p, half, core, ncore, core_blocks, s);
+  }
+  if (nband > 0) hipLaunchKernelGGL((k_bandb<ST, KT, CT>), dim3(band_blocks), dim3(64, 4), 0, s, band, nband, half);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_half(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
+                             const BatchBand* band, int nband, int band_blocks, hipStream_t s) {
+  switch (p.precision) {
+    case TW_FP64: return run_batch_half<double, double, double>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
+    case TW_FP32: return run_batch_half<float, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
+    case TW_BF16:
+      if (p.coef16) return run_batch_half<bf16, bf16, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
+      return run_batch_half<bf16, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
+    default:
+      if (p.coef16)
+        return run_batch_half<_Float16, _Float16, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
+      return run_batch_half<_Float16, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
+  }
+}
+
+// batch items of one problem for one half step (host): core geometry and
+// band rectangles as the single-problem launch uses them
+void batch_geometry(const Plan& p, BatchCore* core, int* core_blocks, BatchBand* band, int* band_blocks) {
+  Core C;
+  Rects R;
+  core_geometry(p, &C, &R);
+  dim3 g, b;
+  grid_dims(p, &g, &b);
+  core->c = C;
+  core->nbx = (int32_t)g.x;
+  core->nb = (C.rhi > C.rlo && C.chi > C.clo) ? (int32_t)(g.x * g.y) : 0;
+  *core_blocks = core->nb;
+  R.poff = (int32_t)(g.x * g.y);
+  band->r = R;
+  band->nb = R.cum[R.n];
+  *band_blocks = band->nb;
 }
 
 // ------------------------------------------------------------ adjoint

@@ -103,6 +103,12 @@ _SIGS = [
     ("tw_check_finite", ctypes.c_int, [_vp]),
     ("tw_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     ("tw_adjoint_run", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    ("tw_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(_vp)]),
+    ("tw_batch_destroy", ctypes.c_int, [_vp]),
+    ("tw_batch_last_error", ctypes.c_char_p, [_vp]),
+    ("tw_batch_step", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("tw_batch_set_rows", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("tw_batch_autotune", ctypes.c_int, [_vp, ctypes.c_int32, _ip]),
 ]
 
 
@@ -337,3 +343,43 @@ class Solver:
     def check_finite(self):
         e = lib().tw_check_finite(self.h)
         return e == 0 if e in (0, 5) else self._chk(e)
+
+
+class Batch:
+    """Independent solvers advanced together (tw_batch_*): one launch per half
+    step for all of them, the time loop captured in a hipGraph.  Keeps
+    references to the solvers; batched steps record no energy or receivers."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (_vp * len(self.solvers))(*[s.h for s in self.solvers])
+        self.h = _vp()
+        e = lib().tw_batch_create(arr, len(self.solvers), ctypes.byref(self.h))
+        if e:
+            raise TwError(e, lib().tw_global_error().decode())
+
+    def _chk(self, e):
+        if e:
+            raise TwError(e, lib().tw_batch_last_error(self.h).decode())
+
+    def step(self, n):
+        self._chk(lib().tw_batch_step(self.h, int(n)))
+
+    def set_rows(self, rows):
+        self._chk(lib().tw_batch_set_rows(self.h, int(rows)))
+
+    def autotune(self, trial_steps=20):
+        best = ctypes.c_int32()
+        self._chk(lib().tw_batch_autotune(self.h, int(trial_steps), ctypes.byref(best)))
+        return best.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().tw_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
