@@ -141,6 +141,54 @@ def cpu_baseline(m, seconds_target=15.0):
                       "%d steps (%.1f s) incl. per-step FP64 energy" % (m, m, n, t)}
 
 
+def c4_batch(args):
+    """Batched acoustic shots: 64 problems of n x n cells, n uniform in
+    [512, 2048] (seeded), dx = 10 m, each with its own smooth random velocity
+    (1.5-4.5 km/s, Gaussian-filtered noise, 100 m) and a 10-25 Hz Ricker source
+    at a random interior point, characteristic sides, BF16 storage with FP32
+    arithmetic; one batched launch per half step, the 1500-step loop in a
+    hipGraph, tile height from the autotuner.  value = cell updates/s of the
+    whole batch over the graph's device time."""
+    import torch
+    import twave as T
+    import twave.workloads as W
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(4)
+    nprob, K = args.shots, args.c4_steps
+    dx = 0.01
+    vmax = 4.5
+    sizes = [int(v) for v in rng.integers(512, 2049, nprob)]
+    solvers = []
+    for i, n in enumerate(sizes):
+        rho, vp, _ = W.smooth_acoustic_model(n, dx, seed=1000 + i)
+        cfg = T.config(physics=T.ACOUSTIC, mx=n, my=n, dx=dx, num_dx=1, den_dx=100,
+                       dt=0.5 * dx / (math.sqrt(2.0) * vmax), bc="CCCC", precision=T.PRECISION[args.precision],
+                       cpst="0", coef16=0, nt_max=K + 64, record_energy=0, stencil_form=1)
+        s = T.Solver(cfg)
+        s.set_model(rho, vp, vp, 1, 100)
+        f0 = float(rng.uniform(10.0, 25.0))
+        ix, iy = (int(v) for v in rng.integers(n // 8, n - n // 8, 2))
+        s.set_source_ricker(T.GRID_MM, ix, iy, f0, 1.25 / f0)
+        solvers.append(s)
+    B = T.Batch(solvers)
+    rows = B.autotune(20)
+    B.step(K)
+    ms = B.elapsed_ms()
+    cells = sum(n * n for n in sizes)
+    value = cells * K / (ms / 1e3)
+    Bu = 2 * 3 * {0: 8, 1: 4, 2: 2, 3: 2}[T.PRECISION[args.precision]] + 3 * 4
+    out = {"metric": "grid-pt updates/s (batched acoustic shots)", "value": value, "unit": "grid-point updates/s",
+           "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "64 acoustic shots, n x n with n uniform in [512, 2048], dx 10 m, smooth random "
+                                  "velocity 1.5-4.5 km/s, Ricker 10-25 Hz, CCCC SAT, storage %s" % args.precision,
+                      "problems": nprob, "cells": cells, "tile_rows": rows},
+           "roofline": {"bound": "hbm", "achieved": value * Bu / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": value * Bu / 1e9 / HBM_PEAK_GBS, "traffic": None, "bytes_per_update": Bu,
+                        "kernel": "one step = batched velocity + stress launches of all problems"}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +203,10 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=0, help="grid of the CPU sample (default: --size)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
+    ap.add_argument("--workload", default="elastic", choices=["elastic", "c4"],
+                    help="elastic: the mixed-precision elastic case (default); c4: batched acoustic shots")
+    ap.add_argument("--shots", type=int, default=64, help="c4: number of problems")
+    ap.add_argument("--c4-steps", type=int, default=1500, help="c4: time steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (the slab decomposition "
@@ -163,6 +215,9 @@ def main():
     import torch
     if args.copy_calib:
         copy_calibration(torch, args.copy_calib, 5)
+        return
+    if args.workload == "c4":
+        c4_batch(args)
         return
     import twave as T
     import twave.workloads as W

@@ -253,6 +253,8 @@ int tw_batch_destroy(tw_batch* b);
 const char* tw_batch_last_error(const tw_batch* b);
 /* Advance every solver of the batch by nsteps. */
 int tw_batch_step(tw_batch* b, int32_t nsteps);
+/* Device time (ms) of the last tw_batch_step's graph launch; waits for it. */
+int tw_batch_elapsed(tw_batch* b, double* ms);
 /* Rows per block of the core launches (the tile height). */
 int tw_batch_set_rows(tw_batch* b, int32_t rows);
 /* Time trial_steps batched steps for each candidate tile height (4..64 rows),

@@ -1296,6 +1296,7 @@ struct tw_batch {
   int rows = 16;  // chunk height (rows per block) of the core launches
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the last graph launch
   // cached graph of the last step range
   hipGraphExec_t exec = nullptr;
   int g_nsteps = -1, g_rows = -1;
@@ -1489,7 +1490,8 @@ int tw_batch_create(tw_solver* const* solvers, int32_t n, tw_batch** out) {
   b->sv.assign(solvers, solvers + n);
   b->device = c0.device;
   b->rows = solvers[0]->plan.txc;
-  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
     tw_batch_destroy(b);
     return fail(nullptr, TW_EDEVICE, "batch: stream or event creation failed");
   }
@@ -1504,6 +1506,8 @@ int tw_batch_destroy(tw_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->items) (void)hipFree(b->items);
   if (b->done) (void)hipEventDestroy(b->done);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return TW_OK;
@@ -1523,7 +1527,17 @@ int tw_batch_step(tw_batch* b, int32_t nsteps) {
   (void)hipSetDevice(b->device);
   int e;
   if (!batch_cached(b, nsteps) && (e = batch_build(b, nsteps))) return e;
-  return batch_launch(b, nsteps, nullptr, nullptr);
+  return batch_launch(b, nsteps, b->ev0, b->ev1);
+}
+
+int tw_batch_elapsed(tw_batch* b, double* ms) {
+  if (!b || !ms) return bfail(b, TW_EINVAL, "bad argument");
+  (void)hipSetDevice(b->device);
+  float t = 0.f;
+  BHIP(b, hipEventSynchronize(b->ev1));
+  BHIP(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
+  *ms = t;
+  return TW_OK;
 }
 
 int tw_batch_autotune(tw_batch* b, int32_t trial_steps, int32_t* best_rows) {

@@ -108,6 +108,7 @@ _SIGS = [
     ("tw_batch_last_error", ctypes.c_char_p, [_vp]),
     ("tw_batch_step", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("tw_batch_set_rows", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("tw_batch_elapsed", ctypes.c_int, [_vp, _dp]),
     ("tw_batch_autotune", ctypes.c_int, [_vp, ctypes.c_int32, _ip]),
 ]
 
@@ -364,6 +365,12 @@ class Batch:
 
     def step(self, n):
         self._chk(lib().tw_batch_step(self.h, int(n)))
+
+    def elapsed_ms(self):
+        """Device time of the last step() graph launch (waits for it)."""
+        ms = ctypes.c_double()
+        self._chk(lib().tw_batch_elapsed(self.h, ctypes.byref(ms)))
+        return ms.value
 
     def set_rows(self, rows):
         self._chk(lib().tw_batch_set_rows(self.h, int(rows)))
