@@ -16,12 +16,14 @@ import twave as T
 pytestmark = pytest.mark.gpu
 
 # Kernel variants: streaming kernels with the default chunk of 16 rows, with a
-# 5-row chunk (many chunk boundaries and window warm-ups), and the per-point
-# kernels.  Read by tw_create from the environment.
-KERNELS = [("2", "16"), ("2", "5"), ("1", "16")]
+# 5-row chunk (many chunk boundaries and window warm-ups), the per-point
+# kernels, and the fused single-pass step (elastic, SBP on both axes; other
+# configurations fall back to the streaming kernels) with both chunk sizes.
+# Read by tw_create from the environment.
+KERNELS = [("2", "16"), ("2", "5"), ("1", "16"), ("3", "16"), ("3", "5")]
 
 
-@pytest.fixture(params=KERNELS, ids=["stream", "stream-txc5", "point"], autouse=True)
+@pytest.fixture(params=KERNELS, ids=["stream", "stream-txc5", "point", "fused", "fused-txc5"], autouse=True)
 def kernel_variant(request):
     old = (os.environ.get("TW_KERNEL"), os.environ.get("TW_TXC"))
     os.environ["TW_KERNEL"], os.environ["TW_TXC"] = request.param

@@ -13,6 +13,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <utility>
 #include <array>
 #include <cmath>
 #include <string>
@@ -413,6 +414,16 @@ void set_default_material(tw_solver* s) {
   }
 }
 
+// the fused step's buffers: the written one becomes current
+void swap_buffers(Plan& p) {
+  for (int f = 0; f < tw::NF; f++) {
+    std::swap(p.f[f].s, p.f[f].s2);
+    std::swap(p.f[f].r, p.f[f].r2);
+  }
+  p.parity ^= 1;
+  p.dstat = p.dstat_ip[p.parity];
+}
+
 tw::Pending stress_pending(tw_solver* s, int it_done, bool energy) {
   tw::Pending p;
   const tw_config* c = &s->cfg;
@@ -521,7 +532,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   memcpy(p.bc, c->bc, 4);
   p.energy_form = c->energy_form;
   p.dx = c->dx;
-  if (const char* kv = getenv("TW_KERNEL")) p.kver = atoi(kv) == 1 ? 1 : 2;
+  if (const char* kv = getenv("TW_KERNEL")) p.kver = atoi(kv) == 1 ? 1 : (atoi(kv) == 3 ? 3 : 2);
   if (const char* tx = getenv("TW_TXC")) p.txc = atoi(tx) > 0 ? atoi(tx) : 16;
   int pad = 0;
   if (const char* pp = getenv("TW_PITCH_PAD")) pad = atoi(pp) > 0 ? (atoi(pp) + 3) / 4 * 4 : 0;
@@ -541,6 +552,15 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     if (rmode(c->cpst) && (e = dalloc(s, &d.r, bytes))) return bail(e);
     if (cmode(c->cpst) && (e = dalloc(s, &d.c, bytes))) return bail(e);
   }
+  p.fused = p.kver == 3 && tw::fused_ok(p);
+  if (p.fused)
+    for (int f = 0; f < 6; f++) {  // the other buffer of each ping-pong pair
+      if (!field_in(c->physics, f)) continue;
+      tw::DevField& d = p.f[f];
+      const size_t bytes = (size_t)d.rows * d.pitch * es;
+      if ((e = dalloc(s, &d.s2, bytes))) return bail(e);
+      if (rmode(c->cpst) && (e = dalloc(s, &d.r2, bytes))) return bail(e);
+    }
   const size_t ks = coef_size(c->precision, c->coef16);
   for (int k = 0; k < 6; k++) {
     if (!coef_in(c->physics, k)) continue;
@@ -573,21 +593,28 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   if ((e = dalloc(s, (void**)&p.ek, nt * 8)) || (e = dalloc(s, (void**)&p.ep0, nt * 8)) ||
       (e = dalloc(s, (void**)&p.ep1, nt * 8)))
     return bail(e);
-  p.nparts = tw::half_parts(p);
+  p.nparts = p.fused ? tw::fused_parts(p) : tw::half_parts(p);
   const size_t npart_max = (size_t)std::max(p.nparts, tw::energy_parts(p));
   if ((e = dalloc(s, (void**)&p.part_v, npart_max * 8)) || (e = dalloc(s, (void**)&p.part_s, npart_max * 8)))
     return bail(e);
-  {
-    const size_t sb = tw::stat_fill(p, nullptr);
-    std::vector<uint8_t> hb(sb);
-    tw::stat_fill(p, hb.data());
-    if ((e = dalloc(s, &p.dstat, sb))) return bail(e);
-    he = hipMemcpy(p.dstat, hb.data(), sb, hipMemcpyHostToDevice);
-    if (he != hipSuccess) {
-      hip_fail(s, he, "hipMemcpy");
-      return bail(TW_EDEVICE);
+  // device copies of the static kernel arguments: in place, and (fused step)
+  // read-current / write-other, for each parity of the ping-pong pairs
+  for (int par = 0; par < (p.fused ? 2 : 1); par++) {
+    for (int pp = 0; pp < (p.fused ? 2 : 1); pp++) {
+      const size_t sb = tw::stat_fill(p, nullptr, pp);
+      std::vector<uint8_t> hb(sb);
+      tw::stat_fill(p, hb.data(), pp);
+      void** dst = pp ? &p.dstat_pp[par] : &p.dstat_ip[par];
+      if ((e = dalloc(s, dst, sb))) return bail(e);
+      he = hipMemcpy(*dst, hb.data(), sb, hipMemcpyHostToDevice);
+      if (he != hipSuccess) {
+        hip_fail(s, he, "hipMemcpy");
+        return bail(TW_EDEVICE);
+      }
     }
+    if (p.fused) swap_buffers(p);
   }
+  p.dstat = p.dstat_ip[p.parity];
   he = hipStreamCreate(&s->stream);
   if (he != hipSuccess) {
     hip_fail(s, he, "hipStreamCreate");
@@ -768,6 +795,62 @@ int tw_step(tw_solver* s, int32_t nsteps) {
   const bool vsrc = s->src_grid == TW_GRID_MN || s->src_grid == TW_GRID_NM;
   tw::Pending carry;  // stress work of the previous step, done by the next launch
   bool have_carry = false;
+  if (p.fused) {
+    for (int k = 0; k < nsteps; k++) {
+      const int it = s->it;
+      const bool rec = c->record_energy && it < c->nt_max;
+      if (rec && !s->ep_valid) {
+        HIPCHK(s, tw::launch_stress_energy(p, s->stream));
+        carry = tw::Pending();
+        carry.part = p.part_s;
+        carry.nparts = tw::energy_parts(p);
+        carry.out0 = p.ep0;
+        carry.it0 = it;
+        carry.scale = c->energy_form == 1 ? 1.0 : 0.5;
+        have_carry = true;
+        s->ep_valid = true;
+      }
+      if (s->need_uhat) HIPCHK(s, tw::launch_uhat(p, s->stream));
+      const bool src = s->src_grid >= 0 && it < (int)s->src_inc.size();
+      auto ctl = [&](bool on) {
+        tw::StepCtl x;
+        x.it = it;
+        x.energy = rec;
+        if (on) {
+          x.src_on = 1;
+          x.src_grid = s->src_grid;
+          x.src_ix = s->src_ix;
+          x.src_iy = s->src_iy;
+          x.src_inc = s->src_inc[it];
+        }
+        return x;
+      };
+      tw::StepCtl cv = ctl(src && vsrc);
+      if (have_carry) cv.pend = carry;
+      HIPCHK(s, tw::launch_fused(p, 0, cv, s->stream));
+      HIPCHK(s, tw::launch_fused(p, 1, ctl(src), s->stream));
+      tw::StepCtl cs = ctl(src && !vsrc);
+      if (rec) {
+        cs.pend.part = p.part_v;
+        cs.pend.nparts = p.nparts;
+        cs.pend.out0 = p.ek;
+        cs.pend.it0 = it;
+        cs.pend.scale = c->energy_form == 1 ? 1.0 : 0.5;
+      }
+      if (it < c->nt_max && !s->rcv_v.empty()) {
+        cs.pend.rcv = p.rcv_v;
+        cs.pend.nrcv = (int32_t)s->rcv_v.size();
+        cs.pend.rcv_it = it;
+      }
+      HIPCHK(s, tw::launch_fused(p, 2, cs, s->stream));
+      carry = stress_pending(s, it, rec);
+      have_carry = true;
+      swap_buffers(p);
+      s->it++;
+    }
+    if (have_carry) HIPCHK(s, tw::launch_finalize(p, carry, s->stream));
+    return TW_OK;
+  }
   for (int k = 0; k < nsteps; k++) {
     const int it = s->it;
     const bool rec = c->record_energy && it < c->nt_max;

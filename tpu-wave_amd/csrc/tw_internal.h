@@ -15,6 +15,8 @@ struct DevField {
   void* s = nullptr;  // state
   void* r = nullptr;  // R companion (3R/-3R/6R residual, or the 16-bit Kahan residual)
   void* c = nullptr;  // C companion (3C/-3C/6C)
+  void* s2 = nullptr; // fused step: the other buffer of the state's ping-pong pair
+  void* r2 = nullptr; // and of the R companion's
   int32_t rows = 0, cols = 0;
   int64_t pitch = 0;  // elements
 };
@@ -66,8 +68,12 @@ struct Plan {
   int32_t perx, pery;
   char bc[4];
   int32_t energy_form;
-  int32_t kver = 2;   // 1: per-point kernels, 2: streaming kernels
-  void* dstat = nullptr;  // device copy of the static kernel arguments
+  int32_t kver = 2;   // 1: per-point kernels, 2: streaming kernels, 3: fused step
+  int32_t fused = 0;  // fused ping-pong step in use
+  int32_t parity = 0; // which buffer of each ping-pong pair is current
+  void* dstat = nullptr;  // device copy of the static kernel arguments (in place, current buffers)
+  void* dstat_ip[2] = {}; // in-place arguments for each parity
+  void* dstat_pp[2] = {}; // fused-step arguments (read current, write other) for each parity
   int32_t txc = 16;   // rows per chunk of the streaming kernels (TW_TXC overrides)
   double dx;
   Tables t;
@@ -131,6 +137,7 @@ struct Dyn {
   int32_t src_on, src_grid, src_ix, src_iy;
   double src_inc;
   double* part;
+  double* part2;  // fused step: stress-energy partials (part: velocity)
   Pending pend;
   double* rec;
   int32_t nt_max;
@@ -180,8 +187,14 @@ struct BatchBand {
 void grid_dims(const Plan& p, dim3* grid, dim3* block);
 int energy_parts(const Plan& p);  // partial sums written by launch_stress_energy
 int half_parts(const Plan& p);    // partial sums written per half step
-// bytes of the static kernel arguments; fills `out` (host memory) when non-null
-size_t stat_fill(const Plan& p, void* out);
+// bytes of the static kernel arguments; fills `out` (host memory) when non-null.
+// pp: updated values go to the other buffers (s2, r2) instead of in place
+size_t stat_fill(const Plan& p, void* out, int pp = 0);
+// fused elastic step: eligible configuration, partial sums per half step,
+// and the three launches of a step (0 band velocity, 1 fused core, 2 band stress)
+bool fused_ok(const Plan& p);
+int fused_parts(const Plan& p);
+hipError_t launch_fused(const Plan& p, int what, const StepCtl& c, hipStream_t s);
 hipError_t launch_half(const Plan& p, int half, const StepCtl& c, hipStream_t s);
 hipError_t launch_uhat(const Plan& p, hipStream_t s);
 hipError_t launch_stress_energy(const Plan& p, hipStream_t s);

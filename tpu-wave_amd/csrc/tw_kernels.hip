@@ -85,9 +85,11 @@ struct TabT {
 
 template <class ST>
 struct F {
-  ST* s;
+  ST* s;   // state read by the launch
   ST* r;
   ST* c;
+  ST* so;  // where updated values go: s itself, or the other buffer of a
+  ST* ro;  // ping-pong pair (the fused step reads one buffer, writes the other)
   int32_t rows, cols;
   int64_t pitch;
   __device__ __forceinline__ int64_t at(int ix, int iy) const { return (int64_t)ix * pitch + iy; }
@@ -128,6 +130,8 @@ static Args<ST, KT, CT> make_args(const Plan& p, const StepCtl* c, double* part)
     a.f[i].s = (ST*)p.f[i].s;
     a.f[i].r = (ST*)p.f[i].r;
     a.f[i].c = (ST*)p.f[i].c;
+    a.f[i].so = a.f[i].s;
+    a.f[i].ro = a.f[i].r;
     a.f[i].rows = p.f[i].rows;
     a.f[i].cols = p.f[i].cols;
     a.f[i].pitch = p.f[i].pitch;
@@ -172,6 +176,7 @@ static Args<ST, KT, CT> make_args(const Plan& p, const StepCtl* c, double* part)
   a.rec = p.rec;
   a.nt_max = p.nt_max;
   a.part = part;
+  a.part2 = nullptr;
   if (c) {
     a.it = c->it;
     a.energy = c->energy;
@@ -549,8 +554,8 @@ __device__ __forceinline__ void vel_pt(const Stat<ST, KT, CT>& A, const Dyn& D, 
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
       if (D.src_on && D.src_grid == TW_GRID_NM && ix == D.src_ix && iy == D.src_iy)
         add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
-      X.s[q] = sn;
-      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.r[q] = rn;
+      X.so[q] = sn;
+      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.ro[q] = rn;
       if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 1, 0, ix, iy, X.rows, X.cols);
     }
   }
@@ -576,8 +581,8 @@ __device__ __forceinline__ void vel_pt(const Stat<ST, KT, CT>& A, const Dyn& D, 
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
       if (D.src_on && D.src_grid == TW_GRID_MN && ix == D.src_ix && iy == D.src_iy)
         add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
-      X.s[q] = sn;
-      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.r[q] = rn;
+      X.so[q] = sn;
+      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.ro[q] = rn;
       if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 0, 1, ix, iy, X.rows, X.cols);
     }
   }
@@ -619,8 +624,8 @@ __device__ __forceinline__ void str_pt(const Stat<ST, KT, CT>& A, const Dyn& D, 
         update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
         if (D.src_on && D.src_grid == TW_GRID_NN && ix == D.src_ix && iy == D.src_iy)
           add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
-        X.s[q] = sn;
-        if (rmode) X.r[q] = rn;
+        X.so[q] = sn;
+        if (rmode) X.ro[q] = rn;
         if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), mu, 1, 1, ix, iy, X.rows, X.cols);
       }
     }
@@ -652,11 +657,11 @@ __device__ __forceinline__ void str_pt(const Stat<ST, KT, CT>& A, const Dyn& D, 
           add_source<ST, CT>(&s0, &r0, D.src_inc, kmode);
           add_source<ST, CT>(&s1, &r1, D.src_inc, kmode);
         }
-        X0.s[q] = s0;
-        X1.s[q] = s1;
+        X0.so[q] = s0;
+        X1.so[q] = s1;
         if (rmode) {
-          X0.r[q] = r0;
-          X1.r[q] = r1;
+          X0.ro[q] = r0;
+          X1.ro[q] = r1;
         }
         if (D.energy) {
           const double L = (double)lam, M2 = (double)tmu;
@@ -686,8 +691,8 @@ __device__ __forceinline__ void str_pt(const Stat<ST, KT, CT>& A, const Dyn& D, 
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
       if (D.src_on && D.src_grid == TW_GRID_MM && ix == D.src_ix && iy == D.src_iy)
         add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
-      X.s[q] = sn;
-      if (rmode) X.r[q] = rn;
+      X.so[q] = sn;
+      if (rmode) X.ro[q] = rn;
       if (D.energy) {
         const CT v = value_of<ST, CT>(sn, rn, kmode);
         if (A.eform == 1) e += energy_pt(A, v, kap, 0, 0, ix, iy, X.rows, X.cols);
@@ -970,17 +975,23 @@ template <class ST>
 struct FR {
   gptr<ST> s;
   gptr<ST> r;
+  gptr<ST> so;
+  gptr<ST> ro;
   int64_t pitch;
   int32_t rows, cols;
   __device__ __forceinline__ void load(const F<ST>& f) {
     s = as_global(f.s);
     r = as_global(f.r);
+    so = as_global(f.so);
+    ro = as_global(f.ro);
     pitch = f.pitch;
     rows = f.rows;
     cols = f.cols;
   }
   __device__ __forceinline__ gptr<ST> row(int i) const { return s + (int64_t)i * pitch; }
   __device__ __forceinline__ gptr<ST> rrow(int i) const { return r + (int64_t)i * pitch; }
+  __device__ __forceinline__ gptr<ST> orow(int i) const { return so + (int64_t)i * pitch; }
+  __device__ __forceinline__ gptr<ST> orrow(int i) const { return ro + (int64_t)i * pitch; }
 };
 
 template <class ST, class CT>
@@ -2074,26 +2085,36 @@ static hipError_t run_half(const Plan& p, int half, const StepCtl& c, hipStream_
 }
 
 template <class ST, class KT, class CT>
-static size_t stat_fill_t(const Plan& p, void* out) {
+static void to_other(const Plan& p, Stat<ST, KT, CT>& a) {
+  for (int i = 0; i < NF; i++) {
+    a.f[i].so = (ST*)p.f[i].s2;
+    a.f[i].ro = (ST*)p.f[i].r2;
+  }
+}
+
+template <class ST, class KT, class CT>
+static size_t stat_fill_t(const Plan& p, void* out, int pp) {
   if (out) {
-    const Stat<ST, KT, CT> st = make_args<ST, KT, CT>(p, nullptr, nullptr);
+    Stat<ST, KT, CT> st = make_args<ST, KT, CT>(p, nullptr, nullptr);
+    if (pp) to_other(p, st);
     memcpy(out, &st, sizeof st);
   }
   return sizeof(Stat<ST, KT, CT>);
 }
 
-size_t stat_fill(const Plan& p, void* out) {
+size_t stat_fill(const Plan& p, void* out, int pp) {
   switch (p.precision) {
-    case TW_FP64: return stat_fill_t<double, double, double>(p, out);
-    case TW_FP32: return stat_fill_t<float, float, float>(p, out);
+    case TW_FP64: return stat_fill_t<double, double, double>(p, out, pp);
+    case TW_FP32: return stat_fill_t<float, float, float>(p, out, pp);
     case TW_BF16:
-      if (p.coef16) return stat_fill_t<bf16, bf16, float>(p, out);
-      return stat_fill_t<bf16, float, float>(p, out);
+      if (p.coef16) return stat_fill_t<bf16, bf16, float>(p, out, pp);
+      return stat_fill_t<bf16, float, float>(p, out, pp);
     default:
-      if (p.coef16) return stat_fill_t<_Float16, _Float16, float>(p, out);
-      return stat_fill_t<_Float16, float, float>(p, out);
+      if (p.coef16) return stat_fill_t<_Float16, _Float16, float>(p, out, pp);
+      return stat_fill_t<_Float16, float, float>(p, out, pp);
   }
 }
+
 
 template <class ST, class KT, class CT>
 static hipError_t run_misc(const Plan& p, int what, const Pending* pend, hipStream_t s) {
@@ -2151,24 +2172,391 @@ hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s) {
     dim3 g((d.cols + 63) / 64, (d.rows + 3) / 4);
     switch (p.precision) {
       case TW_FP64: {
-        F<double> X{(double*)d.s, nullptr, nullptr, d.rows, d.cols, d.pitch};
+        F<double> X{(double*)d.s, nullptr, nullptr, nullptr, nullptr, d.rows, d.cols, d.pitch};
         hipLaunchKernelGGL(k_finite<double>, g, b, 0, s, X, flag);
       } break;
       case TW_FP32: {
-        F<float> X{(float*)d.s, nullptr, nullptr, d.rows, d.cols, d.pitch};
+        F<float> X{(float*)d.s, nullptr, nullptr, nullptr, nullptr, d.rows, d.cols, d.pitch};
         hipLaunchKernelGGL(k_finite<float>, g, b, 0, s, X, flag);
       } break;
       case TW_BF16: {
-        F<bf16> X{(bf16*)d.s, nullptr, nullptr, d.rows, d.cols, d.pitch};
+        F<bf16> X{(bf16*)d.s, nullptr, nullptr, nullptr, nullptr, d.rows, d.cols, d.pitch};
         hipLaunchKernelGGL(k_finite<bf16>, g, b, 0, s, X, flag);
       } break;
       default: {
-        F<_Float16> X{(_Float16*)d.s, nullptr, nullptr, d.rows, d.cols, d.pitch};
+        F<_Float16> X{(_Float16*)d.s, nullptr, nullptr, nullptr, nullptr, d.rows, d.cols, d.pitch};
         hipLaunchKernelGGL(k_finite<_Float16>, g, b, 0, s, X, flag);
       } break;
     }
   }
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------ fused step
+// One pass of the elastic leapfrog step over the interior, both axes SBP: a
+// wave walks down the rows of its strip computing the velocity of row v and,
+// two rows behind it, the stress of row s = v - 2 from the new velocities
+// held in registers, so each field is read once and written once per step.
+// Strips overlap: all 64 lanes compute velocities, lanes 1..62 (248 columns)
+// own the outputs, so the stress y-derivatives of owned lanes find every new
+// velocity they need inside the wave.  Chunks of rows recompute two velocity
+// rows on each side.  The launch reads the old state (s pointers) and writes
+// the other buffer of the ping-pong pair (so pointers); velocities of the SBP
+// band rows and columns come from the band launch that ran before, in the
+// new buffer.  Energies of the owned points go to part (velocity) and part2
+// (stress), unweighted and scaled by dx*dx per block.
+constexpr int FSTRIDE = STRIP - 2 * VW;
+
+template <class ST, class KT, class CT, int CP>
+__device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
+                                           int by, int nbx) {
+  constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
+  using S = St<ST>;
+  __shared__ __attribute__((aligned(16))) CT segA[SEG];  // old sxy row v (y partner of vx)
+  __shared__ __attribute__((aligned(16))) CT segB[SEG];  // old syy row v (y partner of vy)
+  __shared__ __attribute__((aligned(16))) CT segC[SEG];  // new vx row s (y partner of sxy)
+  __shared__ __attribute__((aligned(16))) CT segD[SEG];  // new vy row s (y partner of MM)
+  __shared__ double sh[8];
+  const Stat<ST, KT, CT>& A = *Sp;
+  FR<ST> VX, VY, SXX, SYY, SXY;
+  VX.load(A.f[TW_VX]);
+  VY.load(A.f[TW_VY]);
+  SXX.load(A.f[TW_SXX]);
+  SYY.load(A.f[TW_SYY]);
+  SXY.load(A.f[TW_SXY]);
+  const KT* bxp = A.k[TW_C_BINV_VX].p;
+  const int64_t bxs = A.k[TW_C_BINV_VX].pitch;
+  const KT* byp = A.k[TW_C_BINV_VY].p;
+  const int64_t bys = A.k[TW_C_BINV_VY].pitch;
+  const KT* lap = A.k[TW_C_LAMBDA].p;
+  const int64_t las = A.k[TW_C_LAMBDA].pitch;
+  const KT* tmp_ = A.k[TW_C_TWOMU].p;
+  const int64_t tms = A.k[TW_C_TWOMU].pitch;
+  const KT* mup = A.k[TW_C_MU].p;
+  const int64_t mus = A.k[TW_C_MU].pitch;
+  CT st[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) st[k] = A.t.st[k];
+  const int my = A.my;
+  const double dx = A.dx;
+  const int energy = D.energy;
+
+  const int lane = threadIdx.x;
+  const int base = bx * FSTRIDE;
+  const int c = base + lane * VW;
+  const int cmax = (my + 63) / 64 * 64 - 4;
+  const int cl = min(c, cmax);
+  const int hc = halo_col(lane, base, cl, cmax);
+  const bool hl = lane < 2 || lane >= 62;
+  const int hs = lane < 2 ? lane * 4 : HALO + STRIP + (lane - 62) * 4;
+  const int x0 = C.rlo + by * C.txc;
+  const int x1 = min(x0 + C.txc, C.rhi);
+  bool own[4], vb[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int col = c + j;
+    own[j] = lane >= 1 && lane <= 62 && col >= C.clo && col < C.chi;
+    vb[j] = col < C.clo || col >= C.chi;
+  }
+  const bool own4 = own[0] && own[3];
+  const bool edge = base < C.clo || base + STRIP > C.chi;  // strip has band columns
+
+  CT OA[4][4], OB[4][4], NX[4][4], NY[4][4];
+  V4<ST> SQ[4];
+  const int v0 = x0 - 2;
+  ldx_f<true, ST, CT>(SXX, v0 - 2, cl, 0, 0, OA[1]);
+  ldx_f<true, ST, CT>(SXX, v0 - 1, cl, 0, 0, OA[2]);
+  ldx_f<true, ST, CT>(SXX, v0, cl, 0, 0, OA[3]);
+  ldx_f<true, ST, CT>(SXY, v0 - 2, cl, 0, 0, OB[0]);
+  ldx_f<true, ST, CT>(SXY, v0 - 1, cl, 0, 0, OB[1]);
+  ldx_f<true, ST, CT>(SXY, v0, cl, 0, 0, OB[2]);
+  ldx_f<true, ST, CT>(SXY, v0 + 1, cl, 0, 0, OB[3]);
+  double ev = 0.0, es = 0.0;
+
+  // roles: oaF <- sxx row v+1, oa0..oa2 = rows v-2..v; e = sxy row v-2,
+  // b1..b3 = rows v-1..v+1; nxF <- vx row v, nx0..nx2 = rows v-3..v-1;
+  // ny0..ny3 = vy rows v-4..v-1; sqF <- raw syy row v, sqc = row v-2
+  auto body = [&](int v, CT(&oaF)[4], CT(&oa0)[4], CT(&oa1)[4], CT(&oa2)[4], CT(&e)[4], CT(&b1)[4], CT(&b2)[4],
+                  CT(&b3)[4], CT(&nxF)[4], CT(&nx0)[4], CT(&nx1)[4], CT(&nx2)[4], CT(&ny0)[4], CT(&ny1)[4],
+                  CT(&ny2)[4], CT(&ny3)[4], V4<ST>& sqF, V4<ST>& sqc) {
+    const int s = v - 2;
+    const bool do_s = s >= x0 && s < x1;
+    const bool vcore = v >= C.rlo && v < C.rhi;
+    const bool vown = v >= x0 && v < x1;
+    const int sr = s < x0 ? x0 : s;  // stress row of the loads
+    // ---- loads
+    const V4<ST> nA = ld_v4<ST>(SXX.row(v + 1) + cl);
+    const V4<ST> nB = ld_v4<ST>(SXY.row(v + 2) + cl);
+    const V4<ST> hA = ld_v4<ST>(SXY.row(v) + hc);
+    sqF = ld_v4<ST>(SYY.row(v) + cl);
+    const V4<ST> hB = ld_v4<ST>(SYY.row(v) + hc);
+    const V4<KT> kx = ld_v4<KT>(bxp + (int64_t)v * bxs + cl);
+    V4<ST> sx = ld_v4<ST>(VX.row(v) + cl), qx = sx;
+    if constexpr (RM) qx = ld_v4<ST>(VX.rrow(v) + cl);
+    const V4<KT> ky = ld_v4<KT>(byp + (int64_t)v * bys + cl);
+    V4<ST> sy = ld_v4<ST>(VY.row(v) + cl), qy = sy;
+    if constexpr (RM) qy = ld_v4<ST>(VY.rrow(v) + cl);
+    const V4<KT> km = ld_v4<KT>(mup + (int64_t)sr * mus + cl);
+    const V4<KT> k0 = ld_v4<KT>(lap + (int64_t)sr * las + cl);
+    const V4<KT> k1 = ld_v4<KT>(tmp_ + (int64_t)sr * tms + cl);
+    V4<ST> qn, q0, q1;
+    if constexpr (RM) {
+      qn = ld_v4<ST>(SXY.rrow(sr) + cl);
+      q0 = ld_v4<ST>(SXX.rrow(sr) + cl);
+      q1 = ld_v4<ST>(SYY.rrow(sr) + cl);
+    }
+    V4<ST> bvx, bvy;
+    if (!vcore || edge) {  // the band launch's new velocities
+      bvx = ld_v4<ST>(VX.orow(v) + cl);
+      bvy = ld_v4<ST>(VY.orow(v) + cl);
+    }
+    // ---- velocity row v
+    cvt_v4<ST, CT>(nA, oaF);
+    CT tB[4];
+    cvt_v4<ST, CT>(nB, tB);
+    seg_own<CT>(segA, b2, lane);
+    if (hl) seg_put4<ST, CT>(segA + hs, hA);
+    {
+      CT o[4];
+      cvt_v4<ST, CT>(sqF, o);
+      seg_own<CT>(segB, o, lane);
+    }
+    if (hl) seg_put4<ST, CT>(segB + hs, hB);
+    __syncthreads();
+    CT bxv[4], byv[4];
+    cvt_v4<KT, CT>(kx, bxv);
+    cvt_v4<KT, CT>(ky, byv);
+    if (vcore) {
+      CT ya[12], yb[12];
+      seg12<CT>(segA, lane, ya);
+      seg12<CT>(segB, lane, yb);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        CT acc = RM ? (CT)S::ld(qx.v[j]) : (CT)0;
+        acc = acc + dx4<CT>(oa0, oa1, oa2, oaF, j, st) * bxv[j];
+        acc = acc + dy12<CT>(ya, j, 0, st) * bxv[j];
+        upd<ST, CT, CP>(sx.v[j], qx.v[j], acc);
+        CT acy = RM ? (CT)S::ld(qy.v[j]) : (CT)0;
+        acy = acy + dx4<CT>(b1, b2, b3, tB, j, st) * byv[j];
+        acy = acy + dy12<CT>(yb, j, 1, st) * byv[j];
+        upd<ST, CT, CP>(sy.v[j], qy.v[j], acy);
+      }
+      src4<ST, CT, CP>(sx, qx, D, TW_GRID_NM, v, c);
+      src4<ST, CT, CP>(sy, qy, D, TW_GRID_MN, v, c);
+      if (edge) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          sx.v[j] = vb[j] ? bvx.v[j] : sx.v[j];
+          sy.v[j] = vb[j] ? bvy.v[j] : sy.v[j];
+        }
+      }
+    } else {
+      sx = bvx;
+      sy = bvy;
+    }
+    if (vown) {
+      if (energy) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          ev += own[j] ? e_single<CT>(value_of<ST, CT>(sx.v[j], qx.v[j], KM), bxv[j]) +
+                             e_single<CT>(value_of<ST, CT>(sy.v[j], qy.v[j], KM), byv[j])
+                       : 0.0;
+      }
+      st_m<ST>(VX.orow(v), c, sx, own4, own);
+      st_m<ST>(VY.orow(v), c, sy, own4, own);
+      if constexpr (RM) {
+        st_m<ST>(VX.orrow(v), c, qx, own4, own);
+        st_m<ST>(VY.orrow(v), c, qy, own4, own);
+      }
+    }
+    cvt_v4<ST, CT>(sx, nxF);
+    CT vyv[4];
+    cvt_v4<ST, CT>(sy, vyv);
+    // ---- stress row s = v - 2
+    if (do_s) {
+      seg_own<CT>(segC, nx1, lane);
+      seg_own<CT>(segD, ny2, lane);
+      __syncthreads();
+      {  // sxy (NN)
+        CT mu[4], ya[12];
+        cvt_v4<KT, CT>(km, mu);
+        seg12<CT>(segC, lane, ya);
+        V4<ST> sn;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          sn.v[j] = S::st(e[j]);
+          CT acc = RM ? (CT)S::ld(qn.v[j]) : (CT)0;
+          acc = acc + dx4<CT>(ny0, ny1, ny2, ny3, j, st) * mu[j];
+          acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
+          upd<ST, CT, CP>(sn.v[j], qn.v[j], acc);
+        }
+        src4<ST, CT, CP>(sn, qn, D, TW_GRID_NN, s, c);
+        if (energy) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) es += own[j] ? e_single<CT>(value_of<ST, CT>(sn.v[j], qn.v[j], KM), mu[j]) : 0.0;
+        }
+        st_m<ST>(SXY.orow(s), c, sn, own4, own);
+        if constexpr (RM) st_m<ST>(SXY.orrow(s), c, qn, own4, own);
+      }
+      {  // sxx, syy (MM)
+        CT lam[4], tmu[4], yb[12];
+        cvt_v4<KT, CT>(k0, lam);
+        cvt_v4<KT, CT>(k1, tmu);
+        seg12<CT>(segD, lane, yb);
+        V4<ST> s0, s1 = sqc;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          s0.v[j] = S::st(oa0[j]);
+          CT a0 = RM ? (CT)S::ld(q0.v[j]) : (CT)0;
+          CT a1 = RM ? (CT)S::ld(q1.v[j]) : (CT)0;
+          const CT vv = dx4<CT>(nx0, nx1, nx2, nxF, j, st);
+          a0 = a0 + vv * lam[j];
+          a1 = a1 + vv * lam[j];
+          a0 = a0 + vv * tmu[j];
+          const CT w = dy12<CT>(yb, j, 0, st);
+          a0 = a0 + w * lam[j];
+          a1 = a1 + w * lam[j];
+          a1 = a1 + w * tmu[j];
+          upd<ST, CT, CP>(s0.v[j], q0.v[j], a0);
+          upd<ST, CT, CP>(s1.v[j], q1.v[j], a1);
+        }
+        src4<ST, CT, CP>(s0, q0, D, TW_GRID_MM, s, c);
+        src4<ST, CT, CP>(s1, q1, D, TW_GRID_MM, s, c);
+        if (energy) {
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            es += own[j] ? e_mm<CT>(lam[j], tmu[j], value_of<ST, CT>(s0.v[j], q0.v[j], KM),
+                                    value_of<ST, CT>(s1.v[j], q1.v[j], KM))
+                         : 0.0;
+        }
+        st_m<ST>(SXX.orow(s), c, s0, own4, own);
+        st_m<ST>(SYY.orow(s), c, s1, own4, own);
+        if constexpr (RM) {
+          st_m<ST>(SXX.orrow(s), c, q0, own4, own);
+          st_m<ST>(SYY.orrow(s), c, q1, own4, own);
+        }
+      }
+    }
+    // ---- window updates that the stress row still needed
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      e[j] = tB[j];
+      ny0[j] = vyv[j];
+    }
+  };
+  for (int v = v0;;) {
+    if (v > x1 + 1) break;
+    body(v, OA[0], OA[1], OA[2], OA[3], OB[0], OB[1], OB[2], OB[3], NX[0], NX[1], NX[2], NX[3], NY[0], NY[1], NY[2],
+         NY[3], SQ[0], SQ[2]);
+    if (++v > x1 + 1) break;
+    body(v, OA[1], OA[2], OA[3], OA[0], OB[1], OB[2], OB[3], OB[0], NX[1], NX[2], NX[3], NX[0], NY[1], NY[2], NY[3],
+         NY[0], SQ[1], SQ[3]);
+    if (++v > x1 + 1) break;
+    body(v, OA[2], OA[3], OA[0], OA[1], OB[2], OB[3], OB[0], OB[1], NX[2], NX[3], NX[0], NX[1], NY[2], NY[3], NY[0],
+         NY[1], SQ[2], SQ[0]);
+    if (++v > x1 + 1) break;
+    body(v, OA[3], OA[0], OA[1], OA[2], OB[3], OB[0], OB[1], OB[2], NX[3], NX[0], NX[1], NX[2], NY[3], NY[0], NY[1],
+         NY[2], SQ[3], SQ[1]);
+    ++v;
+  }
+  if (energy) {
+    const double bv = block_sum(ev * (dx * dx), sh);
+    const double bs = block_sum(es * (dx * dx), sh);
+    if (lane == 0) {
+      D.part[by * nbx + bx] = bv;
+      D.part2[by * nbx + bx] = bs;
+    }
+  }
+}
+
+template <class ST, class KT, class CT, int CP>
+__global__ void __launch_bounds__(64) k_fused(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
+  fused_body<ST, KT, CT, CP>(Sp, D, C, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+// ---- fused step launches
+bool fused_ok(const Plan& p) {
+  return p.physics == TW_ELASTIC && !p.perx && !p.pery && p.energy_form == 0 && p.mx >= 12 && p.my >= 12 &&
+         !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C);
+}
+
+static void fused_dims(const Plan& p, const Core& C, dim3* g) {
+  *g = dim3((unsigned)((C.chi - C.clo + FSTRIDE - 1) / FSTRIDE), (unsigned)((C.rhi - C.rlo + p.txc - 1) / p.txc), 1);
+}
+
+int fused_parts(const Plan& p) {
+  Core C;
+  Rects R;
+  core_geometry(p, &C, &R);
+  dim3 g;
+  fused_dims(p, C, &g);
+  return (int)(g.x * g.y) + R.cum[R.n];
+}
+
+template <class ST, class KT, class CT>
+static hipError_t run_fused(const Plan& p, int what, const StepCtl& c, hipStream_t s) {
+  Core C;
+  Rects R;
+  core_geometry(p, &C, &R);
+  dim3 g;
+  fused_dims(p, C, &g);
+  R.poff = (int32_t)(g.x * g.y);
+  if (what == 1) {
+    auto a = make_args<ST, KT, CT>(p, &c, p.part_v);
+    const Dyn d = a;
+    Dyn dd = d;
+    dd.part2 = p.part_s;
+    const Stat<ST, KT, CT>* sp = (const Stat<ST, KT, CT>*)p.dstat_pp[p.parity];
+    auto go = [&](auto cp) {
+      constexpr int CP = decltype(cp)::value;
+      hipLaunchKernelGGL((k_fused<ST, KT, CT, CP>), g, dim3(64), 0, s, sp, dd, C);
+    };
+    if constexpr (sizeof(ST) >= 4) {
+      switch (p.cpst) {
+        case TW_CPST_3R: go(std::integral_constant<int, TW_CPST_3R>{}); break;
+        case TW_CPST_M3R: go(std::integral_constant<int, TW_CPST_M3R>{}); break;
+        case TW_CPST_6R: go(std::integral_constant<int, TW_CPST_6R>{}); break;
+        default: go(std::integral_constant<int, TW_CPST_0>{}); break;
+      }
+    } else {
+      if (p.cpst == TW_CPST_K)
+        go(std::integral_constant<int, TW_CPST_K>{});
+      else
+        go(std::integral_constant<int, TW_CPST_0>{});
+    }
+    return hipGetLastError();
+  }
+  // band launches: the velocity band reads the old state and writes the new
+  // velocities; the stress band reads the new velocities and the old stress
+  // and writes the new stress
+  auto a = make_args<ST, KT, CT>(p, &c, what == 0 ? p.part_v : p.part_s);
+  to_other(p, a);
+  if (what == 2) {
+    a.f[TW_VX].s = (ST*)p.f[TW_VX].s2;
+    a.f[TW_VX].r = (ST*)p.f[TW_VX].r2;
+    a.f[TW_VY].s = (ST*)p.f[TW_VY].s2;
+    a.f[TW_VY].r = (ST*)p.f[TW_VY].r2;
+  }
+  if (R.n == 0) return hipSuccess;
+  const int nb = R.cum[R.n];
+  if (what == 0)
+    hipLaunchKernelGGL((k_velocity<ST, KT, CT>), dim3(nb), dim3(64, 4), 0, s, a, R);
+  else
+    hipLaunchKernelGGL((k_stress<ST, KT, CT>), dim3(nb), dim3(64, 4), 0, s, a, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused(const Plan& p, int what, const StepCtl& c, hipStream_t s) {
+  switch (p.precision) {
+    case TW_FP64: return run_fused<double, double, double>(p, what, c, s);
+    case TW_FP32: return run_fused<float, float, float>(p, what, c, s);
+    case TW_BF16:
+      if (p.coef16) return run_fused<bf16, bf16, float>(p, what, c, s);
+      return run_fused<bf16, float, float>(p, what, c, s);
+    default:
+      if (p.coef16) return run_fused<_Float16, _Float16, float>(p, what, c, s);
+      return run_fused<_Float16, float, float>(p, what, c, s);
+  }
 }
 
 // ------------------------------------------------------------ batched shots
