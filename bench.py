@@ -76,13 +76,21 @@ def measured_traffic(key):
 
 def copy_calibration(torch, nbytes, reps):
     """Plain device copy of nbytes, reps times: the known-size kernel that
-    scales the FETCH_SIZE / WRITE_SIZE counters."""
+    scales the FETCH_SIZE / WRITE_SIZE counters, and the attainable
+    read+write bandwidth of a streaming copy on this device."""
     a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
     b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(reps):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps + 1, "copy_read_plus_write_GBps": 2 * nbytes / t / 1e9}))
 
 
 def make_solver(T, W, m, precision, cpst, coef16, nt, model_from=None):
