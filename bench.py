@@ -44,24 +44,38 @@ def algorithmic_bytes(precision, cpst, coef16, physics=0):
     return 2 * F * s + 2 * F * c + K * p
 
 
+def half_bytes(precision, cpst, coef16, half, physics=0):
+    """Compulsory HBM bytes per cell of one half step's own fields: the state
+    and stored compensation of the fields it updates (read + write) and its
+    coefficients (read).  Velocity (half 0): 2 fields, 2 coefficient arrays;
+    stress (half 1): 3 fields and 3 arrays elastic, 1 and 1 acoustic.  The two
+    halves sum to algorithmic_bytes."""
+    F = 2 if half == 0 else (3 if physics == 0 else 1)
+    s = {0: 8, 1: 4, 2: 2, 3: 2}[precision]
+    comp = cpst in ("3R", "-3R", "6R", "3C", "-3C", "6C", "K")
+    c = s if comp else 0
+    p = 8 if precision == 0 else (2 if coef16 else 4)
+    return 2 * F * s + 2 * F * c + F * p
+
+
 def kernel_source_hash():
-    """SHA-256 (first 16 hex digits) of the HIP sources: a measured traffic
-    figure is used only for the kernels it was measured on."""
+    """SHA-256 (first 16 hex digits) of the kernel sources (the HIP kernels and
+    launches, and the plan they read): a measured traffic figure is used only
+    for the kernels it was measured on."""
     import hashlib
     h = hashlib.sha256()
     d = os.path.join(ROOT, "tpu-wave_amd", "csrc")
-    for name in sorted(os.listdir(d)):
-        if name.endswith((".hip", ".h", ".cpp")):
-            with open(os.path.join(d, name), "rb") as f:
-                h.update(f.read())
+    for name in ("tw_internal.h", "tw_kernels.hip"):
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(f.read())
     return h.hexdigest()[:16]
 
 
-def measured_traffic(key):
-    """HBM bytes per step measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE,
-    calibrated by a copy kernel), from profiles/traffic.txt lines
-    'traffic <workload-key> <kernel-hash> <bytes-per-step>'; None if absent
-    or measured on other kernel sources."""
+def measured_traffic(key, kernel):
+    """HBM bytes per launch of `kernel` ('step' for a whole step) measured with
+    rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated by a copy kernel), from
+    profiles/traffic.txt lines 'traffic <workload-key> <kernel-hash> <kernel>
+    <bytes>'; None if absent or measured on other kernel sources."""
     path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
@@ -69,8 +83,8 @@ def measured_traffic(key):
     with open(path) as f:
         for line in f:
             t = line.split()
-            if len(t) == 4 and t[0] == "traffic" and t[1] == key and t[2] == kh:
-                return float(t[3])
+            if len(t) == 5 and t[0] == "traffic" and t[1] == key and t[2] == kh and t[3] == kernel:
+                return float(t[4])
     return None
 
 
@@ -103,13 +117,18 @@ def make_solver(T, W, m, precision, cpst, coef16, nt, model_from=None):
     return s
 
 
-def timed_steps(torch, s, steps, warmup):
+def timed_steps(torch, s, steps, warmup, profile=False):
+    """Device time (s) of `steps` steps on the solver's stream and the wall
+    time; with profile, also the summed durations (ms) of the velocity and
+    stress core launches inside the timed region (HIP events on that stream)."""
     stream = torch.cuda.Stream()
     s.set_stream(stream.cuda_stream)
     if warmup:
         s.step(warmup)
     s.synchronize()
     torch.cuda.synchronize()
+    if profile:
+        s.profile_begin(steps)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -119,7 +138,8 @@ def timed_steps(torch, s, steps, warmup):
     s.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return e0.elapsed_time(e1) / 1e3, wall
+    prof = s.profile_end() if profile else None
+    return e0.elapsed_time(e1) / 1e3, wall, prof
 
 
 def cpu_baseline(m, seconds_target=15.0):
@@ -208,6 +228,8 @@ def main():
     ap.add_argument("--coef16", type=int, default=1, help="store material coefficients in the 16-bit type")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32 and FP64 runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no per-launch events in the timed region (roofline from the step time)")
     ap.add_argument("--cpu-size", type=int, default=0, help="grid of the CPU sample (default: --size)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
@@ -235,12 +257,30 @@ def main():
     prec = T.PRECISION[args.precision]
 
     s = make_solver(T, W, m, prec, args.cpst, args.coef16, nt)
-    t_gpu, wall = timed_steps(torch, s, K, Wn)
+    t_gpu, wall, prof = timed_steps(torch, s, K, Wn, profile=not args.no_profile)
     value = m * m * K / t_gpu
     B = algorithmic_bytes(prec, args.cpst, args.coef16)
-    achieved = B * m * m / (t_gpu / K) / 1e9
     key = "elastic-%d-%s-%s-%d" % (m, args.precision, args.cpst, args.coef16)
-    traffic = measured_traffic(key)
+    step_ach = B * m * m / (t_gpu / K) / 1e9
+    step = {"achieved": step_ach, "frac": step_ach / HBM_PEAK_GBS, "bytes_per_update": B,
+            "traffic": measured_traffic(key, "step")}
+    if prof:
+        # dominant kernel: the core launch of the half step with the larger device time
+        ms_v, ms_s, n = prof
+        half = 1 if ms_s >= ms_v else 0
+        name = ("k_str3", "k_vel3")[half == 0]
+        us = (ms_s if half else ms_v) / n * 1e3
+        Bh = half_bytes(prec, args.cpst, args.coef16, half)
+        achieved = Bh * m * m / (us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(key, name),
+                "kernel": name + " (%s half step, core region)" % ("stress" if half else "velocity"),
+                "launch_us": us, "bytes_per_cell": Bh, "other_core_launch_us": (ms_v if half else ms_s) / n * 1e3,
+                "traffic_unit": "HBM bytes per launch", "traffic_key": key, "kernel_hash": kernel_source_hash(),
+                "step": step}
+    else:
+        roof = dict(step, bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",
+                    kernel="one step = velocity + stress half-step kernels")
     out = {
         "metric": METRIC,
         "value": value,
@@ -259,18 +299,14 @@ def main():
                                (m, m, args.precision, args.cpst, args.coef16),
                    "cells": m * m, "fields": 5, "storage": args.precision, "cpst": args.cpst,
                    "coef16": args.coef16},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "one step = velocity + stress half-step kernels",
-                     "bytes_per_update": B, "traffic_unit": "HBM bytes per step",
-                     "traffic_key": key, "kernel_hash": kernel_source_hash()},
+        "roofline": roof,
     }
     if not args.no_compare:
         fields_lp = {f: s.get_field(f) for f in s.fields}
         rec_lp = s.receivers()
         # same (quantized) material as the 16-bit run, so the three runs differ only in storage
         s32 = make_solver(T, W, m, T.FP32, "0", 0, nt, model_from=s)
-        t32, _ = timed_steps(torch, s32, K, Wn)
+        t32, _, _ = timed_steps(torch, s32, K, Wn)
         out["fp32_value"] = m * m * K / t32
         out["speedup_vs_fp32"] = out["value"] / out["fp32_value"]
         fields32 = {f: s32.get_field(f) for f in s32.fields}

@@ -214,6 +214,116 @@ int tw_set_stream(tw_solver* s, void* hip_stream);
 /* Check the state for non-finite values (TW_ENONFINITE if any). */
 int tw_check_finite(tw_solver* s);
 
+/* Configuration the solver was created with. */
+int tw_get_config(const tw_solver* s, tw_config* cfg);
+
+/* ---- reference-format front-end (host only) ----
+ * The reference reads its problem from text files in a key_value_value format
+ * (cuda/src/cpp/input_processing.cpp:11-137), lets command-line options
+ * override them (:140-200), reads sources and receivers from SrcRcv.txt
+ * (cuda/src/cpp/inverse.cpp:133-348) and the material from raw FP64 files
+ * (inverse.cpp:6-130), and writes one Signal_*.txt per receiver and one
+ * Energy_*.txt per source (cuda/src/cpp/forward.cpp:215-327).  These entry
+ * points read and write the same formats.  Axis convention as above: the
+ * files' physical X is data axis 1 (y), physical Y is data axis 0 (x). */
+typedef struct tw_input {
+  int32_t model_m[2];              /* model cells along data x, y   (model_<X>_<Y>) */
+  int32_t grid_m[2];               /* simulation cells along data x, y (grids_<X>_<Y>) */
+  int64_t num_dx_model, den_dx_model; /* model spacing num/den (dxM_<num>_<den>)    */
+  int64_t num_dx, den_dx;          /* simulation spacing num/den (dxS_<num>_<den>)  */
+  char    bc[4];                   /* xL, xR, yL, yR as tw_config.bc (bdry_<XL><XR><YL><YR>,
+                                      physical X first)                              */
+  int32_t nt;                      /* Nt_                                            */
+  double  dt_max;                  /* MAXdt_: dt = dt_max * cfl                      */
+  double  cfl;                     /* CFL_                                           */
+  double  central_f;               /* frequency_ (Ricker)                            */
+  double  time_delay;              /* delay_ (Ricker t0)                             */
+  int32_t device;                  /* device_                                        */
+  char    energy;                  /* 'Y' or 'N' (option -energy)                    */
+  char    medium[64];              /* option -medium: the data folder name           */
+} tw_input;
+
+/* Reference defaults (cuda/src/cpp/namespace_input.hpp:13-68): all sides 'F',
+ * Nt -1, dt_max 2^30, CFL 0.625, f 5, t0 0.25, energy 'Y', device -1,
+ * medium "homogeneous"; sizes and spacings unset (-1). */
+void tw_input_default(tw_input* in);
+/* Parse an InputFile.txt (file_input_processing): lines whose first
+ * '_'-separated token is a known key set the value; other lines are ignored.
+ * TW_EINVAL if the file cannot be opened or a value does not parse. */
+int tw_read_input_file(const char* path, tw_input* in);
+/* Apply one command-line option as command_line_input_processing does:
+ * name is one of medium, B_type, Nt, dt_max, CFL, energy, f, t0, device. */
+int tw_input_option(tw_input* in, const char* name, const char* value);
+/* checking_and_printouts_input_parameters + ns_input_derived_variables
+ * (input_processing.cpp:203-214, namespace_input.hpp:124-146) and the
+ * numerical parameters of main_host_FDM_2D.tpp:60-61: checks that model and
+ * simulation sizes describe the same extent and that energy is 'Y' or 'N',
+ * then fills cfg (defaults of tw_config_default for everything the file does
+ * not set) with sizes, dx = num/den, dt = dt_max * cfl, boundaries, nt_max = Nt,
+ * device, and the given physics, precision and compensation. */
+int tw_input_to_config(const tw_input* in, int32_t physics, int32_t precision, int32_t cpst, tw_config* cfg);
+/* Note: the reference's elastic solver runs periodic along data y and with
+ * free-surface SAT along data x whatever the letters say (it requires them
+ * all to be 'F', namespace_input.hpp:158-160, and differentiates y
+ * periodically, grid_drvt.tpp:1-3); the letters are taken here as written, so a
+ * caller reproducing a reference run sets cfg->bc to "FFPP" afterwards. */
+
+typedef struct tw_srcrcv_entry {
+  int32_t kind;       /* 0 source (Src_ line), 1 receiver (Rcv_ line)            */
+  int32_t src_index;  /* source index (a receiver's source)                     */
+  int32_t rcv_index;  /* receiver index within its source; -1 for sources        */
+  char    grid[2];    /* grid type along data x, y ('N' or 'M')                  */
+  int64_t num[2];     /* location along data x, y as num/den in length units     */
+  int64_t den[2];
+} tw_srcrcv_entry;
+
+/* Parse a SrcRcv.txt (Class_Inverse_Specs::input_SrcRcv_locations): entries in
+ * file order.  With out == NULL only *n is set.  Errors as the reference's
+ * checks: a receiver before its source, duplicate source locations or indices,
+ * duplicate receiver locations or indices within a source (TW_EINVAL); more
+ * entries than cap (TW_ERANGE, *n = the count needed). */
+int tw_read_srcrcv_file(const char* path, tw_srcrcv_entry* out, int32_t cap, int32_t* n);
+
+/* Read a raw FP64 model file (rho, vp or vs) of (mx_model+1) x (my_model+1)
+ * nodes, data x outer (Class_Inverse_Specs::input_inverse_parameter,
+ * inverse.cpp:6-130): the byte size must match exactly; on an axis whose two
+ * sides are 'P' in bc the first and last node lines must be equal. */
+int tw_read_model_file(const char* path, int32_t mx_model, int32_t my_model, const char bc[4], double* out);
+
+/* Set up one shot of a SrcRcv list on a solver (process_src_locations and
+ * process_rcv_locations, forward.cpp:10-211): a Ricker source (central_f,
+ * time_delay of in) at the source's location, and every field of each of its
+ * receivers' grids recorded, receivers in file order, fields in the
+ * reference's order per grid (NN: sxy; MN: vy; NM: vx; MM: sxx then syy, or p).
+ * The checks of tw_set_source_ricker and tw_set_receivers apply (no source on
+ * a weak node boundary, forward.cpp:95-98); a source index that is not in the
+ * list or has no receivers is refused (TW_EINVAL, forward.cpp main loop
+ * main_host_FDM_2D.tpp:203-205). */
+int tw_setup_shot(tw_solver* s, const tw_input* in, const tw_srcrcv_entry* entries, int32_t n, int32_t src_index);
+/* Write the results of the shot set up with tw_setup_shot after tw_step:
+ * Signal_src_<i>_<grid>_<iy>_<ix>_rcv_<j>_<grid>_<iy>_<ix>.txt per receiver
+ * (grid letters physical X first; each line "% 6d  " then "% 16.15e  " per
+ * field, scaled by source_scaling) and, when energy is recorded,
+ * Energy_src_<i>_<grid>_<iy>_<ix>.txt with E = E_k + (E_p0 + E_p1)/2 scaled by
+ * source_scaling^2, "%16.15e" per line.  folder must exist. */
+int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* entries, int32_t n, int32_t src_index);
+/* Result folder of the reference's device driver
+ * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:755-772):
+ * <root>/<elastic|acoustic>/<medium>/device/grid_size_<num>_<den>/Nt_<Nt>/
+ * <fp64|fp32|bf16|fp16>/dt_<dt as %16.15e with the '.' as 'p'>/<cpst>, cpst as
+ * 0_0, 3_R, -3_R, 6_R, 3_C, -3_C, 6_C or K.  Writes at most cap bytes. */
+int tw_result_folder(const tw_input* in, const tw_config* cfg, const char* root, char* out, int32_t cap);
+
+/* ---- per-launch timing of the core kernels ----
+ * No counterpart in the reference, whose device driver has no timers
+ * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:400).  After tw_profile_begin,
+ * the next max_steps steps record HIP events on the solver's stream around
+ * the core launch of each half step (velocity, stress; the single core launch
+ * of the fused step counts as velocity).  tw_profile_end waits for the stream
+ * and returns the summed durations (ms) and the number of profiled steps. */
+int tw_profile_begin(tw_solver* s, int32_t max_steps);
+int tw_profile_end(tw_solver* s, double* ms_velocity, double* ms_stress, int32_t* nsteps);
+
 /* ---- discrete adjoint (acoustic; FP64 or FP32 storage; no compensation) ----
  *
  * Gradient of a receiver misfit L(traces) with respect to the material

@@ -5,7 +5,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["csrc/tw_host.cpp", "csrc/tw_kernels.hip"]
+SOURCES = ["csrc/tw_host.cpp", "csrc/tw_io.cpp", "csrc/tw_kernels.hip"]
 OUT = os.path.join(HERE, "twave", "libtwave.so")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -298,6 +298,9 @@ struct tw_solver {
   size_t adj_state_bytes = 0, adj_inj_n = 0;
   int adj_it0 = 0, adj_n = 0, adj_I = 0, adj_nckpt = 0, adj_cap_ckpt = 0, adj_cap_seg = 0;
   bool adj_done = false;     // a checkpointed run is available
+  // per-launch timing: 4 events per profiled step
+  std::vector<hipEvent_t> prof_ev;
+  int prof_n = 0, prof_max = 0;
 };
 
 namespace {
@@ -449,9 +452,17 @@ tw::Pending stress_pending(tw_solver* s, int it_done, bool energy) {
 }  // namespace
 
 // ---------------------------------------------------------------- C-ABI
+int tw::host_fail(int code, const char* msg) { return fail(nullptr, code, msg); }
+
 extern "C" {
 
 const char* tw_global_error(void) { return g_err.c_str(); }
+
+int tw_get_config(const tw_solver* s, tw_config* cfg) {
+  if (!s || !cfg) return fail(nullptr, TW_EINVAL, "null argument");
+  *cfg = s->cfg;
+  return TW_OK;
+}
 const char* tw_version(void) { return "twave 0.1 (gfx950)"; }
 
 void tw_config_default(tw_config* c) {
@@ -633,6 +644,7 @@ int tw_destroy(tw_solver* s) {
   if (!s) return TW_OK;
   (void)hipSetDevice(s->cfg.device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (hipEvent_t e : s->prof_ev) (void)hipEventDestroy(e);
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -828,7 +840,11 @@ int tw_step(tw_solver* s, int32_t nsteps) {
       tw::StepCtl cv = ctl(src && vsrc);
       if (have_carry) cv.pend = carry;
       HIPCHK(s, tw::launch_fused(p, 0, cv, s->stream));
+      const bool prof = s->prof_n < s->prof_max;
+      if (prof) HIPCHK(s, hipEventRecord(s->prof_ev[4 * s->prof_n], s->stream));
       HIPCHK(s, tw::launch_fused(p, 1, ctl(src), s->stream));
+      if (prof) HIPCHK(s, hipEventRecord(s->prof_ev[4 * s->prof_n + 1], s->stream));
+      if (prof) s->prof_n++;
       tw::StepCtl cs = ctl(src && !vsrc);
       if (rec) {
         cs.pend.part = p.part_v;
@@ -877,6 +893,8 @@ int tw_step(tw_solver* s, int32_t nsteps) {
       cv.src_inc = s->src_inc[it];
     }
     if (have_carry) cv.pend = carry;
+    const bool prof = s->prof_n < s->prof_max;
+    for (int i = 0; i < 4; i++) p.prof[i] = prof ? s->prof_ev[4 * s->prof_n + i] : nullptr;
     HIPCHK(s, tw::launch_half(p, 0, cv, s->stream));
     tw::StepCtl cs;
     cs.it = it;
@@ -901,11 +919,56 @@ int tw_step(tw_solver* s, int32_t nsteps) {
       cs.pend.rcv_it = it;
     }
     HIPCHK(s, tw::launch_half(p, 1, cs, s->stream));
+    for (int i = 0; i < 4; i++) p.prof[i] = nullptr;
+    if (prof) s->prof_n++;
     carry = stress_pending(s, it, rec);
     have_carry = true;
     s->it++;
   }
   if (have_carry) HIPCHK(s, tw::launch_finalize(p, carry, s->stream));
+  return TW_OK;
+}
+
+static void prof_release(tw_solver* s) {
+  for (hipEvent_t e : s->prof_ev) (void)hipEventDestroy(e);
+  s->prof_ev.clear();
+  s->prof_n = s->prof_max = 0;
+}
+
+int tw_profile_begin(tw_solver* s, int32_t max_steps) {
+  if (!s || max_steps < 1 || max_steps > (1 << 20)) return fail(s ? &s->err : nullptr, TW_EINVAL, "bad argument");
+  (void)hipSetDevice(s->cfg.device);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  prof_release(s);
+  s->prof_ev.resize((size_t)4 * max_steps, nullptr);
+  for (auto& e : s->prof_ev) {
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) {
+      prof_release(s);
+      return hip_fail(s, r, "hipEventCreate");
+    }
+  }
+  s->prof_max = max_steps;
+  return TW_OK;
+}
+
+int tw_profile_end(tw_solver* s, double* ms_velocity, double* ms_stress, int32_t* nsteps) {
+  if (!s || !ms_velocity || !ms_stress || !nsteps) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  (void)hipSetDevice(s->cfg.device);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  double v = 0.0, st = 0.0;
+  const bool fused = s->plan.fused;
+  for (int k = 0; k < s->prof_n; k++) {
+    float a = 0.f, b = 0.f;
+    HIPCHK(s, hipEventElapsedTime(&a, s->prof_ev[4 * k], s->prof_ev[4 * k + 1]));
+    if (!fused) HIPCHK(s, hipEventElapsedTime(&b, s->prof_ev[4 * k + 2], s->prof_ev[4 * k + 3]));
+    v += a;
+    st += b;
+  }
+  *ms_velocity = v;
+  *ms_stress = st;
+  *nsteps = s->prof_n;
+  prof_release(s);
   return TW_OK;
 }
 

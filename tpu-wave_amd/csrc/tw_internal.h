@@ -10,6 +10,9 @@ namespace tw {
 
 enum { NF = 6 };
 
+// Records msg as the thread's last error (tw_global_error) and returns code.
+int host_fail(int code, const char* msg);
+
 // One field in HBM: row-major [rows][pitch] elements of the storage type.
 struct DevField {
   void* s = nullptr;  // state
@@ -75,6 +78,8 @@ struct Plan {
   void* dstat_ip[2] = {}; // in-place arguments for each parity
   void* dstat_pp[2] = {}; // fused-step arguments (read current, write other) for each parity
   int32_t txc = 16;   // rows per chunk of the streaming kernels (TW_TXC overrides)
+  // timing of this step's core launches: start/stop of velocity, stress (null: off)
+  hipEvent_t prof[4] = {};
   double dx;
   Tables t;
   DevField f[NF];

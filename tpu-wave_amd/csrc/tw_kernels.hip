@@ -2065,8 +2065,10 @@ static hipError_t run_half(const Plan& p, int half, const StepCtl& c, hipStream_
   const Stat<ST, KT, CT>* sp = (const Stat<ST, KT, CT>*)p.dstat;
   const Dyn d = a;
   if (C.rhi > C.rlo && C.chi > C.clo) {
+    if (p.prof[2 * half]) (void)hipEventRecord(p.prof[2 * half], s);
     hipError_t e = launch_core<ST, KT, CT>(p, half, sp, d, C, g, b, s);
     if (e != hipSuccess) return e;
+    if (p.prof[2 * half + 1]) (void)hipEventRecord(p.prof[2 * half + 1], s);
   } else {
     // no core rows: the band launch carries the pending work
     R.poff = 0;

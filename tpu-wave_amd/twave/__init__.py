@@ -55,6 +55,39 @@ class Config(ctypes.Structure):
     ]
 
 
+class Input(ctypes.Structure):
+    """tw_input (include/tw.h): the reference's InputFile.txt parameters."""
+    _fields_ = [
+        ("model_m", ctypes.c_int32 * 2),
+        ("grid_m", ctypes.c_int32 * 2),
+        ("num_dx_model", ctypes.c_int64),
+        ("den_dx_model", ctypes.c_int64),
+        ("num_dx", ctypes.c_int64),
+        ("den_dx", ctypes.c_int64),
+        ("bc", ctypes.c_char * 4),
+        ("nt", ctypes.c_int32),
+        ("dt_max", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("central_f", ctypes.c_double),
+        ("time_delay", ctypes.c_double),
+        ("device", ctypes.c_int32),
+        ("energy", ctypes.c_char),
+        ("medium", ctypes.c_char * 64),
+    ]
+
+
+class SrcRcvEntry(ctypes.Structure):
+    """tw_srcrcv_entry (include/tw.h): one Src_ or Rcv_ line of SrcRcv.txt."""
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("src_index", ctypes.c_int32),
+        ("rcv_index", ctypes.c_int32),
+        ("grid", ctypes.c_char * 2),
+        ("num", ctypes.c_int64 * 2),
+        ("den", ctypes.c_int64 * 2),
+    ]
+
+
 _lib = None
 
 # (name, restype, argtypes)
@@ -101,6 +134,21 @@ _SIGS = [
     ("tw_checkpoint_load", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     ("tw_set_stream", ctypes.c_int, [_vp, _vp]),
     ("tw_check_finite", ctypes.c_int, [_vp]),
+    ("tw_get_config", ctypes.c_int, [_vp, _cp]),
+    ("tw_input_default", None, [ctypes.POINTER(Input)]),
+    ("tw_read_input_file", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Input)]),
+    ("tw_input_option", ctypes.c_int, [ctypes.POINTER(Input), ctypes.c_char_p, ctypes.c_char_p]),
+    ("tw_input_to_config", ctypes.c_int, [ctypes.POINTER(Input), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _cp]),
+    ("tw_read_srcrcv_file", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(SrcRcvEntry), ctypes.c_int32, _ip]),
+    ("tw_read_model_file", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, _dp]),
+    ("tw_setup_shot", ctypes.c_int, [_vp, ctypes.POINTER(Input), ctypes.POINTER(SrcRcvEntry), ctypes.c_int32,
+                                     ctypes.c_int32]),
+    ("tw_write_results", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(SrcRcvEntry), ctypes.c_int32,
+                                        ctypes.c_int32]),
+    ("tw_result_folder", ctypes.c_int, [ctypes.POINTER(Input), _cp, ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.c_int32]),
+    ("tw_profile_begin", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("tw_profile_end", ctypes.c_int, [_vp, _dp, _dp, _ip]),
     ("tw_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     ("tw_adjoint_run", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
     ("tw_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(_vp)]),
@@ -344,6 +392,18 @@ class Solver:
     def check_finite(self):
         e = lib().tw_check_finite(self.h)
         return e == 0 if e in (0, 5) else self._chk(e)
+
+    def profile_begin(self, max_steps):
+        """Time the core launches of the next max_steps steps (HIP events on
+        the solver's stream)."""
+        self._chk(lib().tw_profile_begin(self.h, int(max_steps)))
+
+    def profile_end(self):
+        """(ms of the velocity core launches, ms of the stress core launches,
+        profiled steps), summed over the profiled steps."""
+        v, st, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+        self._chk(lib().tw_profile_end(self.h, ctypes.byref(v), ctypes.byref(st), ctypes.byref(n)))
+        return v.value, st.value, n.value
 
 
 class Batch:
