@@ -314,6 +314,15 @@ int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* en
  * 0_0, 3_R, -3_R, 6_R, 3_C, -3_C, 6_C or K.  Writes at most cap bytes. */
 int tw_result_folder(const tw_input* in, const tw_config* cfg, const char* root, char* out, int32_t cap);
 
+/* Data misfit of the recorded traces against data [n][nt] (the solver's
+ * receiver order, nt <= recorded steps): per record sum over steps of
+ * (trace - data)^2 into per_record[n] (may be NULL) and their sum into
+ * *total.  Replaces Class_Forward_Specs::calculate_data_misfit
+ * (cuda/src/cpp/forward.cpp:414-445), whose per-receiver value is the sum
+ * over the fields of the receiver's grid: with tw_setup_shot's registration a
+ * receiver's fields are consecutive records.  Sums in FP64 in step order. */
+int tw_data_misfit(tw_solver* s, const double* data, int32_t nt, double* per_record, double* total);
+
 /* ---- per-launch timing of the core kernels ----
  * No counterpart in the reference, whose device driver has no timers
  * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:400).  After tw_profile_begin,

@@ -6,6 +6,8 @@ precisions, and through the torch.autograd wrapper.
 The forward solver is itself pinned to the oracle bitwise (test_gpu_parity),
 so finite differences of it are the reference for the transposed step.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -216,3 +218,83 @@ def test_adjoint_c5_scaled_fd_directions():
     _, g32 = P32.misfit(P32.m0, grad=True)
     for a, b in zip(g32, g):
         assert C.rel_l2(a, b) < 1e-3
+
+
+def test_adjoint_shots_workload_small():
+    """The adjoint-gradient workload class at a small size: gradient against
+    central differences on smooth directions (FP64)."""
+    import twave.workloads as W
+    print("case adjoint shots small", flush=True)
+    P = W.AdjointShots(n=64, nt=240, nshots=2, nrcv=32, dx_km=0.04, f0=3.0, precision=T.FP64, start_scale=0.98)
+    J0, g = P.misfit(P.m0, grad=True)
+    assert J0 > 0
+    rng = np.random.default_rng(21)
+    errs = []
+    for _ in range(4):
+        d = [W.smooth_direction(rng, m.shape, 4) * m for m in P.m0]
+        jp, _ = P.misfit([m + 1e-3 * dd for m, dd in zip(P.m0, d)])
+        jm, _ = P.misfit([m - 1e-3 * dd for m, dd in zip(P.m0, d)])
+        fd = (jp - jm) / 2e-3
+        ad = sum(float(np.sum(a * dd)) for a, dd in zip(g, d))
+        errs.append(abs(fd - ad) / abs(fd))
+    P.close()
+    assert max(errs) < 1e-3, errs
+
+
+@pytest.mark.skipif(os.environ.get("TW_RUN_SLOW") != "1", reason="full-scale run (minutes); TW_RUN_SLOW=1")
+def test_adjoint_c5_full_scale():
+    """The adjoint configuration at full size: acoustic 2048^2, 16 Ricker
+    12 Hz shots along the top, 512 receivers, 3000 steps, true model = 2 km/s
+    background + seeded circular 10% anomaly, starting model = background.
+
+    FP64 adjoint gradient against finite differences of the FP64 forward
+    misfit on TW_C5_DIRS (default 32) smooth random directions (direction
+    seed TW_C5_SEED), and the FP32 gradient (sqrt(N) checkpoints) against the
+    FP64 one.  With the receivers beside the sources, the first-order change
+    of the misfit is tiny next to its curvature along the direct wave:
+    central differences at eps = 1e-3 are dominated by their O(eps^2) term
+    (FD / adjoint = 46 at 1e-3, 1.45 at 1e-4, 1.0045 at 1e-5, 1.0006 at 1e-6
+    on the first direction), so the differences are Richardson-extrapolated
+    from h = 1e-5 and 2h (error O(h^4)).  For the same reason the directional
+    derivatives are ~1e-7 of |g| |d|, below FP32 resolution of the gradient,
+    so FP32 is judged by its distance to the FP64 gradient."""
+    import time
+    import twave.workloads as W
+    ndir = int(os.environ.get("TW_C5_DIRS", "32"))
+    seed = int(os.environ.get("TW_C5_SEED", "11"))
+    h = 1e-5
+    log = lambda m: print(m, flush=True)
+    t = time.perf_counter()
+    P64 = W.AdjointShots(precision=T.FP64, log=log)
+    log("fp64 data %.1f s" % (time.perf_counter() - t))
+    t = time.perf_counter()
+    J0, g64 = P64.misfit(P64.m0, grad=True)
+    log("fp64 misfit %.6e, gradient %.1f s" % (J0, time.perf_counter() - t))
+    P32 = W.AdjointShots(precision=T.FP32, data=P64.data)
+    t = time.perf_counter()
+    J32, g32 = P32.misfit(P32.m0, grad=True)
+    t32 = time.perf_counter() - t
+    P32.close()
+    rl = [C.rel_l2(a, b) for a, b in zip(g32, g64)]
+    log("fp32 misfit %.6e, gradient %.1f s; rel-L2 vs fp64 (kappa, 1/rho NM, 1/rho MN) %s" % (J32, t32, rl))
+    rng = np.random.default_rng(seed)
+
+    def cd(d, e):
+        jp, _ = P64.misfit([m + e * dd for m, dd in zip(P64.m0, d)])
+        jm, _ = P64.misfit([m - e * dd for m, dd in zip(P64.m0, d)])
+        return (jp - jm) / (2 * e)
+
+    fds, a64 = [], []
+    for i in range(ndir):
+        d = [W.smooth_direction(rng, m.shape, P64.n / 16) * m for m in P64.m0]
+        d1, d2 = cd(d, h), cd(d, 2 * h)
+        fds.append((4 * d1 - d2) / 3)
+        a64.append(sum(float(np.sum(a * dd)) for a, dd in zip(g64, d)))
+        log("direction %d: cd(h) %.9e cd(2h) %.9e richardson %.9e adjoint64 %.9e" % (i, d1, d2, fds[-1], a64[-1]))
+    P64.close()
+    fds, a64 = np.array(fds), np.array(a64)
+    scale = np.maximum(np.abs(fds), 0.1 * np.median(np.abs(fds)))
+    e64 = np.abs(fds - a64) / scale
+    log("max rel err fp64 adjoint vs finite differences %.3e (median %.3e)" % (e64.max(), np.median(e64)))
+    assert e64.max() < 1e-3
+    assert max(rl) < 1e-3

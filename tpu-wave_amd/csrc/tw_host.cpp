@@ -972,6 +972,28 @@ int tw_profile_end(tw_solver* s, double* ms_velocity, double* ms_stress, int32_t
   return TW_OK;
 }
 
+int tw_data_misfit(tw_solver* s, const double* data, int32_t nt, double* per_record, double* total) {
+  if (!s || (!data && s->nrcv > 0) || !total || nt < 0) return fail(s ? &s->err : nullptr, TW_EINVAL, "bad argument");
+  const int have = s->it < s->cfg.nt_max ? s->it : s->cfg.nt_max;
+  if (nt > have) return fail(&s->err, TW_ERANGE, "more data steps than recorded steps");
+  std::vector<double> rec((size_t)(s->nrcv > 0 ? s->nrcv : 1) * (have > 0 ? have : 1));
+  int32_t got = 0;
+  int e = tw_read_receivers(s, rec.data(), &got);
+  if (e) return e;
+  double sum = 0.0;
+  for (int r = 0; r < s->nrcv; r++) {
+    double m = 0.0;
+    for (int t = 0; t < nt; t++) {
+      const double d = rec[(size_t)r * got + t] - data[(size_t)r * nt + t];
+      m += d * d;
+    }
+    if (per_record) per_record[r] = m;
+    sum += m;
+  }
+  *total = sum;
+  return TW_OK;
+}
+
 int tw_synchronize(tw_solver* s) {
   if (!s) return fail(nullptr, TW_EINVAL, "null handle");
   (void)hipSetDevice(s->cfg.device);

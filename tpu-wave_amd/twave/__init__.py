@@ -147,6 +147,7 @@ _SIGS = [
                                         ctypes.c_int32]),
     ("tw_result_folder", ctypes.c_int, [ctypes.POINTER(Input), _cp, ctypes.c_char_p, ctypes.c_char_p,
                                         ctypes.c_int32]),
+    ("tw_data_misfit", ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp]),
     ("tw_profile_begin", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("tw_profile_end", ctypes.c_int, [_vp, _dp, _dp, _ip]),
     ("tw_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
@@ -392,6 +393,15 @@ class Solver:
     def check_finite(self):
         e = lib().tw_check_finite(self.h)
         return e == 0 if e in (0, 5) else self._chk(e)
+
+    def data_misfit(self, data):
+        """(per-record sum of (trace - data)^2, total) over the first
+        data.shape[1] recorded steps (tw_data_misfit)."""
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        per = np.zeros(self.nrcv)
+        tot = ctypes.c_double()
+        self._chk(lib().tw_data_misfit(self.h, _d(d), d.shape[1], _d(per), ctypes.byref(tot)))
+        return per, tot.value
 
     def profile_begin(self, max_steps):
         """Time the core launches of the next max_steps steps (HIP events on

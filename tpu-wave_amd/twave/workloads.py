@@ -87,3 +87,93 @@ def setup_elastic(solver, m, dx_km, seed=2024, f0=15.0, depth_km=0.1, nrcv=256):
     cols = np.linspace(m // 16, m - m // 16, nrcv).astype(np.int32)
     solver.set_receivers(np.full(nrcv, VX, np.int32), np.zeros(nrcv, np.int32), cols)
     return dict(vpmax=float(vp.max()), src=(ix, iy))
+
+
+class AdjointShots:
+    """The adjoint-gradient configuration: acoustic n x n cells, a uniform
+    background velocity plus a seeded circular 10% anomaly (the true model),
+    Ricker shots on p along row 4 under the free surface (top 'F',
+    characteristic elsewhere), receivers on p along the same row, misfit
+    J = sum over shots and samples of (trace - data)^2, the reference's form
+    without 1/2 (cuda/src/cpp/forward.cpp:437-438), data from the true model.
+
+    m0 is the starting model's (kappa, 1/rho on NM, 1/rho on MN): the
+    background without the anomaly, times `start_scale`."""
+
+    def __init__(self, n=2048, nt=3000, nshots=16, nrcv=512, dx_km=0.01, vbg=2.0, f0=12.0, precision=FP32,
+                 cpst="0", seed=5, start_scale=1.0, log=None, data=None):
+        from . import Solver, P
+        self.n, self.nt, self.log = n, nt, log
+        den = int(round(1.0 / dx_km))
+        dt = 0.4 * dx_km / (math.sqrt(2.0) * 1.1 * vbg)
+        cfg = config(physics=ACOUSTIC, mx=n, my=n, dx=dx_km, num_dx=1, den_dx=den, dt=dt, bc="FCCC",
+                     precision=precision, cpst=cpst, nt_max=nt, stencil_form=1)
+        self.s = Solver(cfg)
+        rng = np.random.default_rng(seed)
+        L = n * dx_km
+        # shallow enough that its reflections reach the surface receivers
+        # within nt steps: centre 8-15% of the width deep, radius 3-5%
+        cx, cy = rng.uniform(0.08, 0.15) * L, rng.uniform(0.35, 0.65) * L
+        rad = rng.uniform(0.03, 0.05) * L
+        X, Y = np.meshgrid(np.arange(n + 1) * dx_km, np.arange(n + 1) * dx_km, indexing="ij")
+        self.rho = np.ones((n + 1, n + 1))
+        self.vp_true = vbg * (1.0 + 0.1 * (((X - cx) ** 2 + (Y - cy) ** 2) < rad ** 2))
+        self.anomaly = (cx, cy, rad)
+        ry = np.linspace(4, n - 5, nrcv).round().astype(np.int32)
+        self.s.set_receivers(np.full(nrcv, P, np.int32), np.full(nrcv, 4, np.int32), ry)
+        self.shots = [int(v) for v in np.linspace(n // 16, n - n // 16, nshots)]
+        self.f0, self.t0 = f0, 1.2 / f0
+        self.state0 = self.s.checkpoint()
+        self.coefs = (C_KAPPA, C_BINV_VX, C_BINV_VY)
+        self.s.set_model(self.rho, self.vp_true, self.vp_true, 1, den)
+        truth = [self.s.get_coef(k) for k in self.coefs]
+        self.data = data
+        if data is None:
+            self.data = []
+            for i in range(nshots):
+                self.data.append(self.run(truth, i)[0])
+                self._log("data shot %d/%d" % (i + 1, nshots))
+        vb = start_scale * vbg * np.ones_like(self.rho)
+        self.s.set_model(self.rho, vb, vb, 1, den)
+        self.m0 = [self.s.get_coef(k) for k in self.coefs]
+
+    def _log(self, msg):
+        if self.log:
+            self.log(msg)
+
+    def run(self, coefs, shot, residual_of=None):
+        """Traces of one shot for the coefficients; with residual_of (the
+        shot's data), also the adjoint gradient of sum (trace - data)^2."""
+        s = self.s
+        s.restore(self.state0)
+        for k, v in zip(self.coefs, coefs):
+            s.set_coef(k, v)
+        s.set_source_ricker(GRID_MM, 4, self.shots[shot], self.f0, self.t0)
+        if residual_of is None:
+            s.step(self.nt)
+            return s.receivers()[:, -self.nt:], None
+        s.adjoint_forward(self.nt)
+        tr = s.receivers()[:, -self.nt:]
+        return tr, s.adjoint_run(2.0 * (tr - residual_of))
+
+    def misfit(self, coefs, grad=False):
+        """J and (with grad) its gradient, summed over the shots."""
+        total, g = 0.0, [np.zeros_like(m) for m in self.m0]
+        for i in range(len(self.shots)):
+            tr, gi = self.run(coefs, i, self.data[i] if grad else None)
+            total += float(np.sum((tr - self.data[i]) ** 2))
+            if grad:
+                for a, b in zip(g, gi):
+                    a += b
+        return total, g
+
+    def close(self):
+        self.s.close()
+
+
+def smooth_direction(rng, shape, corr):
+    """Unit-peak random model perturbation, Gaussian-correlated over `corr`
+    cells."""
+    from scipy.ndimage import gaussian_filter
+    d = gaussian_filter(rng.standard_normal(shape), corr, mode="nearest")
+    return d / np.abs(d).max()
