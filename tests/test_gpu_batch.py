@@ -101,3 +101,10 @@ def test_batch_rejects_mixed():
     b = problem("acoustic", "bf16", "0", 40, 48, "FCCC", 1)
     with pytest.raises(T.TwError):
         T.Batch([a, b])
+
+
+def test_batch_rejects_fused(monkeypatch):
+    monkeypatch.setenv("TW_KERNEL", "3")
+    a = problem("elastic", "bf16", "0", 40, 48, "FFFF", 0)
+    with pytest.raises(T.TwError):
+        T.Batch([a])

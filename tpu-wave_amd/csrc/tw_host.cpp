@@ -1650,6 +1650,7 @@ int tw_batch_create(tw_solver* const* solvers, int32_t n, tw_batch** out) {
         c.device != c0.device || c.energy_form != c0.energy_form)
       return fail(nullptr, TW_EINVAL, "batch: solvers must share physics, precision, coef16, cpst, device, energy form");
     if (s->need_uhat) return fail(nullptr, TW_EINVAL, "batch: characteristic sides of elastic problems are not batched");
+    if (s->plan.fused) return fail(nullptr, TW_EINVAL, "batch: fused-step solvers are not batched");
     if (s->plan.kver == 1 || c.cpst == TW_CPST_3C || c.cpst == TW_CPST_M3C || c.cpst == TW_CPST_6C)
       return fail(nullptr, TW_EINVAL, "batch: streaming kernels only (no 3C/-3C/6C)");
   }

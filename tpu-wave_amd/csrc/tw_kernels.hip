@@ -1227,6 +1227,13 @@ __device__ __forceinline__ void store_f(const FR<ST>& X, int r, int c, const V4<
 template <class ST, int CP>
 constexpr int WPE = sizeof(ST) == 8 ? 1 : (sizeof(ST) == 4 && CP != TW_CPST_0 ? 3 : 4);
 
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_tile(int L, int n) {
+  const int xcd = L % NXCD, local = L / NXCD;
+  const int per = n / NXCD, rem = n % NXCD;
+  return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
 // ---- helpers of the FULL-strip path (strip and its halos inside the row,
 // core rows only): x windows rotate through four register arrays, so the
 // row loop is unrolled by four and the windows are never copied.
@@ -2263,31 +2270,31 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
   const bool own4 = own[0] && own[3];
   const bool edge = base < C.clo || base + STRIP > C.chi;  // strip has band columns
 
-  CT OA[4][4], OB[4][4], NX[4][4], NY[4][4];
-  V4<ST> SQ[4];
+  // x windows: sxx, new vx and syy rows as raw storage values in registers
+  // (rotated by the unrolled loop); sxy and new vy rows in per-lane LDS rings
+  // of five rows (each lane reads back only its own columns)
+  __shared__ __attribute__((aligned(16))) V4<ST> ringB[5][64];  // old sxy rows v-2..v+2
+  __shared__ __attribute__((aligned(16))) V4<ST> ringY[5][64];  // new vy rows v-4..v
+  auto slot5 = [](int r) { return ((r % 5) + 5) % 5; };
+  V4<ST> RA[4], RX[4], SQ[4];
   const int v0 = x0 - 2;
-  ldx_f<true, ST, CT>(SXX, v0 - 2, cl, 0, 0, OA[1]);
-  ldx_f<true, ST, CT>(SXX, v0 - 1, cl, 0, 0, OA[2]);
-  ldx_f<true, ST, CT>(SXX, v0, cl, 0, 0, OA[3]);
-  ldx_f<true, ST, CT>(SXY, v0 - 2, cl, 0, 0, OB[0]);
-  ldx_f<true, ST, CT>(SXY, v0 - 1, cl, 0, 0, OB[1]);
-  ldx_f<true, ST, CT>(SXY, v0, cl, 0, 0, OB[2]);
-  ldx_f<true, ST, CT>(SXY, v0 + 1, cl, 0, 0, OB[3]);
+  RA[1] = ld_v4<ST>(SXX.row(v0 - 2) + cl);
+  RA[2] = ld_v4<ST>(SXX.row(v0 - 1) + cl);
+  RA[3] = ld_v4<ST>(SXX.row(v0) + cl);
+  for (int r = v0 - 2; r <= v0 + 1; r++) ringB[slot5(r)][lane] = ld_v4<ST>(SXY.row(r) + cl);
   double ev = 0.0, es = 0.0;
 
-  // roles: oaF <- sxx row v+1, oa0..oa2 = rows v-2..v; e = sxy row v-2,
-  // b1..b3 = rows v-1..v+1; nxF <- vx row v, nx0..nx2 = rows v-3..v-1;
-  // ny0..ny3 = vy rows v-4..v-1; sqF <- raw syy row v, sqc = row v-2
-  auto body = [&](int v, CT(&oaF)[4], CT(&oa0)[4], CT(&oa1)[4], CT(&oa2)[4], CT(&e)[4], CT(&b1)[4], CT(&b2)[4],
-                  CT(&b3)[4], CT(&nxF)[4], CT(&nx0)[4], CT(&nx1)[4], CT(&nx2)[4], CT(&ny0)[4], CT(&ny1)[4],
-                  CT(&ny2)[4], CT(&ny3)[4], V4<ST>& sqF, V4<ST>& sqc) {
+  // roles: aF <- sxx row v+1, a0..a2 = rows v-2..v; xF <- vx row v,
+  // x0..x2 = rows v-3..v-1; sqF <- syy row v, sqc = row v-2
+  auto body = [&](int v, V4<ST>& aF, V4<ST>& a0, V4<ST>& a1, V4<ST>& a2, V4<ST>& xF, V4<ST>& xr0, V4<ST>& xr1,
+                  V4<ST>& xr2, V4<ST>& sqF, V4<ST>& sqc) {
     const int s = v - 2;
     const bool do_s = s >= x0 && s < x1;
     const bool vcore = v >= C.rlo && v < C.rhi;
     const bool vown = v >= x0 && v < x1;
     const int sr = s < x0 ? x0 : s;  // stress row of the loads
     // ---- loads
-    const V4<ST> nA = ld_v4<ST>(SXX.row(v + 1) + cl);
+    aF = ld_v4<ST>(SXX.row(v + 1) + cl);
     const V4<ST> nB = ld_v4<ST>(SXY.row(v + 2) + cl);
     const V4<ST> hA = ld_v4<ST>(SXY.row(v) + hc);
     sqF = ld_v4<ST>(SYY.row(v) + cl);
@@ -2298,25 +2305,33 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
     const V4<KT> ky = ld_v4<KT>(byp + (int64_t)v * bys + cl);
     V4<ST> sy = ld_v4<ST>(VY.row(v) + cl), qy = sy;
     if constexpr (RM) qy = ld_v4<ST>(VY.rrow(v) + cl);
-    const V4<KT> km = ld_v4<KT>(mup + (int64_t)sr * mus + cl);
-    const V4<KT> k0 = ld_v4<KT>(lap + (int64_t)sr * las + cl);
-    const V4<KT> k1 = ld_v4<KT>(tmp_ + (int64_t)sr * tms + cl);
+    // stress-row loads: issued with the velocity loads when the variant keeps
+    // no residual (its registers allow it); otherwise after the velocity row
+    V4<KT> km, k0, k1;
     V4<ST> qn, q0, q1;
-    if constexpr (RM) {
-      qn = ld_v4<ST>(SXY.rrow(sr) + cl);
-      q0 = ld_v4<ST>(SXX.rrow(sr) + cl);
-      q1 = ld_v4<ST>(SYY.rrow(sr) + cl);
-    }
+    auto ld_stress = [&]() {
+      km = ld_v4<KT>(mup + (int64_t)sr * mus + cl);
+      k0 = ld_v4<KT>(lap + (int64_t)sr * las + cl);
+      k1 = ld_v4<KT>(tmp_ + (int64_t)sr * tms + cl);
+      if constexpr (RM) {
+        qn = ld_v4<ST>(SXY.rrow(sr) + cl);
+        q0 = ld_v4<ST>(SXX.rrow(sr) + cl);
+        q1 = ld_v4<ST>(SYY.rrow(sr) + cl);
+      }
+    };
+    if constexpr (!RM) ld_stress();
     V4<ST> bvx, bvy;
     if (!vcore || edge) {  // the band launch's new velocities
       bvx = ld_v4<ST>(VX.orow(v) + cl);
       bvy = ld_v4<ST>(VY.orow(v) + cl);
     }
     // ---- velocity row v
-    cvt_v4<ST, CT>(nA, oaF);
-    CT tB[4];
-    cvt_v4<ST, CT>(nB, tB);
-    seg_own<CT>(segA, b2, lane);
+    ringB[slot5(v + 2)][lane] = nB;
+    {
+      CT o[4];
+      cvt_v4<ST, CT>(ringB[slot5(v)][lane], o);
+      seg_own<CT>(segA, o, lane);
+    }
     if (hl) seg_put4<ST, CT>(segA + hs, hA);
     {
       CT o[4];
@@ -2332,14 +2347,26 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
       CT ya[12], yb[12];
       seg12<CT>(segA, lane, ya);
       seg12<CT>(segB, lane, yb);
+      CT w0[4], w1[4], w2[4], w3[4];
+      cvt_v4<ST, CT>(a0, w0);
+      cvt_v4<ST, CT>(a1, w1);
+      cvt_v4<ST, CT>(a2, w2);
+      cvt_v4<ST, CT>(aF, w3);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         CT acc = RM ? (CT)S::ld(qx.v[j]) : (CT)0;
-        acc = acc + dx4<CT>(oa0, oa1, oa2, oaF, j, st) * bxv[j];
+        acc = acc + dx4<CT>(w0, w1, w2, w3, j, st) * bxv[j];
         acc = acc + dy12<CT>(ya, j, 0, st) * bxv[j];
         upd<ST, CT, CP>(sx.v[j], qx.v[j], acc);
+      }
+      cvt_v4<ST, CT>(ringB[slot5(v - 1)][lane], w0);
+      cvt_v4<ST, CT>(ringB[slot5(v)][lane], w1);
+      cvt_v4<ST, CT>(ringB[slot5(v + 1)][lane], w2);
+      cvt_v4<ST, CT>(nB, w3);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
         CT acy = RM ? (CT)S::ld(qy.v[j]) : (CT)0;
-        acy = acy + dx4<CT>(b1, b2, b3, tB, j, st) * byv[j];
+        acy = acy + dx4<CT>(w0, w1, w2, w3, j, st) * byv[j];
         acy = acy + dy12<CT>(yb, j, 1, st) * byv[j];
         upd<ST, CT, CP>(sy.v[j], qy.v[j], acy);
       }
@@ -2371,24 +2398,31 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
         st_m<ST>(VY.orrow(v), c, qy, own4, own);
       }
     }
-    cvt_v4<ST, CT>(sx, nxF);
-    CT vyv[4];
-    cvt_v4<ST, CT>(sy, vyv);
+    xF = sx;
     // ---- stress row s = v - 2
     if (do_s) {
-      seg_own<CT>(segC, nx1, lane);
-      seg_own<CT>(segD, ny2, lane);
+      if constexpr (RM) ld_stress();
+      {
+        CT o[4];
+        cvt_v4<ST, CT>(xr1, o);
+        seg_own<CT>(segC, o, lane);
+        cvt_v4<ST, CT>(ringY[slot5(v - 2)][lane], o);
+        seg_own<CT>(segD, o, lane);
+      }
       __syncthreads();
       {  // sxy (NN)
-        CT mu[4], ya[12];
+        CT mu[4], ya[12], w0[4], w1[4], w2[4], w3[4];
         cvt_v4<KT, CT>(km, mu);
         seg12<CT>(segC, lane, ya);
-        V4<ST> sn;
+        cvt_v4<ST, CT>(ringY[slot5(v - 4)][lane], w0);
+        cvt_v4<ST, CT>(ringY[slot5(v - 3)][lane], w1);
+        cvt_v4<ST, CT>(ringY[slot5(v - 2)][lane], w2);
+        cvt_v4<ST, CT>(ringY[slot5(v - 1)][lane], w3);
+        V4<ST> sn = ringB[slot5(s)][lane];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          sn.v[j] = S::st(e[j]);
           CT acc = RM ? (CT)S::ld(qn.v[j]) : (CT)0;
-          acc = acc + dx4<CT>(ny0, ny1, ny2, ny3, j, st) * mu[j];
+          acc = acc + dx4<CT>(w0, w1, w2, w3, j, st) * mu[j];
           acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
           upd<ST, CT, CP>(sn.v[j], qn.v[j], acc);
         }
@@ -2401,26 +2435,29 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
         if constexpr (RM) st_m<ST>(SXY.orrow(s), c, qn, own4, own);
       }
       {  // sxx, syy (MM)
-        CT lam[4], tmu[4], yb[12];
+        CT lam[4], tmu[4], yb[12], w0[4], w1[4], w2[4], w3[4];
         cvt_v4<KT, CT>(k0, lam);
         cvt_v4<KT, CT>(k1, tmu);
         seg12<CT>(segD, lane, yb);
-        V4<ST> s0, s1 = sqc;
+        cvt_v4<ST, CT>(xr0, w0);
+        cvt_v4<ST, CT>(xr1, w1);
+        cvt_v4<ST, CT>(xr2, w2);
+        cvt_v4<ST, CT>(xF, w3);
+        V4<ST> s0 = a0, s1 = sqc;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          s0.v[j] = S::st(oa0[j]);
-          CT a0 = RM ? (CT)S::ld(q0.v[j]) : (CT)0;
-          CT a1 = RM ? (CT)S::ld(q1.v[j]) : (CT)0;
-          const CT vv = dx4<CT>(nx0, nx1, nx2, nxF, j, st);
-          a0 = a0 + vv * lam[j];
-          a1 = a1 + vv * lam[j];
-          a0 = a0 + vv * tmu[j];
+          CT e0 = RM ? (CT)S::ld(q0.v[j]) : (CT)0;
+          CT e1 = RM ? (CT)S::ld(q1.v[j]) : (CT)0;
+          const CT vv = dx4<CT>(w0, w1, w2, w3, j, st);
+          e0 = e0 + vv * lam[j];
+          e1 = e1 + vv * lam[j];
+          e0 = e0 + vv * tmu[j];
           const CT w = dy12<CT>(yb, j, 0, st);
-          a0 = a0 + w * lam[j];
-          a1 = a1 + w * lam[j];
-          a1 = a1 + w * tmu[j];
-          upd<ST, CT, CP>(s0.v[j], q0.v[j], a0);
-          upd<ST, CT, CP>(s1.v[j], q1.v[j], a1);
+          e0 = e0 + w * lam[j];
+          e1 = e1 + w * lam[j];
+          e1 = e1 + w * tmu[j];
+          upd<ST, CT, CP>(s0.v[j], q0.v[j], e0);
+          upd<ST, CT, CP>(s1.v[j], q1.v[j], e1);
         }
         src4<ST, CT, CP>(s0, q0, D, TW_GRID_MM, s, c);
         src4<ST, CT, CP>(s1, q1, D, TW_GRID_MM, s, c);
@@ -2439,26 +2476,18 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
         }
       }
     }
-    // ---- window updates that the stress row still needed
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      e[j] = tB[j];
-      ny0[j] = vyv[j];
-    }
+    // the new vy row joins the ring after the stress row has read rows v-4..v-1
+    ringY[slot5(v)][lane] = sy;
   };
   for (int v = v0;;) {
     if (v > x1 + 1) break;
-    body(v, OA[0], OA[1], OA[2], OA[3], OB[0], OB[1], OB[2], OB[3], NX[0], NX[1], NX[2], NX[3], NY[0], NY[1], NY[2],
-         NY[3], SQ[0], SQ[2]);
+    body(v, RA[0], RA[1], RA[2], RA[3], RX[0], RX[1], RX[2], RX[3], SQ[0], SQ[2]);
     if (++v > x1 + 1) break;
-    body(v, OA[1], OA[2], OA[3], OA[0], OB[1], OB[2], OB[3], OB[0], NX[1], NX[2], NX[3], NX[0], NY[1], NY[2], NY[3],
-         NY[0], SQ[1], SQ[3]);
+    body(v, RA[1], RA[2], RA[3], RA[0], RX[1], RX[2], RX[3], RX[0], SQ[1], SQ[3]);
     if (++v > x1 + 1) break;
-    body(v, OA[2], OA[3], OA[0], OA[1], OB[2], OB[3], OB[0], OB[1], NX[2], NX[3], NX[0], NX[1], NY[2], NY[3], NY[0],
-         NY[1], SQ[2], SQ[0]);
+    body(v, RA[2], RA[3], RA[0], RA[1], RX[2], RX[3], RX[0], RX[1], SQ[2], SQ[0]);
     if (++v > x1 + 1) break;
-    body(v, OA[3], OA[0], OA[1], OA[2], OB[3], OB[0], OB[1], OB[2], NX[3], NX[0], NX[1], NX[2], NY[3], NY[0], NY[1],
-         NY[2], SQ[3], SQ[1]);
+    body(v, RA[3], RA[0], RA[1], RA[2], RX[3], RX[0], RX[1], RX[2], SQ[3], SQ[1]);
     ++v;
   }
   if (energy) {
@@ -2472,8 +2501,9 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
 }
 
 template <class ST, class KT, class CT, int CP>
-__global__ void __launch_bounds__(64) k_fused(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
-  fused_body<ST, KT, CT, CP>(Sp, D, C, blockIdx.x, blockIdx.y, gridDim.x);
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(ST) == 8 ? 1 : (sizeof(ST) == 4 ? 2 : 3)))) k_fused(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C, int nbx) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  fused_body<ST, KT, CT, CP>(Sp, D, C, t % nbx, t / nbx, nbx);
 }
 
 // ---- fused step launches
@@ -2511,7 +2541,7 @@ static hipError_t run_fused(const Plan& p, int what, const StepCtl& c, hipStream
     const Stat<ST, KT, CT>* sp = (const Stat<ST, KT, CT>*)p.dstat_pp[p.parity];
     auto go = [&](auto cp) {
       constexpr int CP = decltype(cp)::value;
-      hipLaunchKernelGGL((k_fused<ST, KT, CT, CP>), g, dim3(64), 0, s, sp, dd, C);
+      hipLaunchKernelGGL((k_fused<ST, KT, CT, CP>), dim3(g.x * g.y), dim3(64), 0, s, sp, dd, C, (int)g.x);
     };
     if constexpr (sizeof(ST) >= 4) {
       switch (p.cpst) {
