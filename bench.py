@@ -150,16 +150,17 @@ def cpu_baseline(m, seconds_target=15.0):
     import twave.workloads as W
     threads = min(16, os.cpu_count() or 1)
     O.set_threads(threads)
-    cfg = W.elastic_config(m=m, precision=T.FP32, nt=8)
+    cap = 64
+    cfg = W.elastic_config(m=m, precision=T.FP32, nt=cap)
     oc = O.make_config(physics=0, mx=m, my=m, dx=cfg.dx, num_dx=cfg.num_dx, den_dx=cfg.den_dx, dt=cfg.dt,
-                       bc="FCCC", precision=O.FP32, nt_max=8)
+                       bc="FCCC", precision=O.FP32, nt_max=cap)
     o = O.Oracle(oc)
     rho, vp, vs = W.layered_elastic_model(m, 0.005)
     o.set_model(rho, vp, vs, 1, 200)
     o.set_source_ricker(O.GRID_NM, 20, m // 2, 15.0, 1.25 / 15.0)
     o.step(1)
     n, t = 0, 0.0
-    while t < seconds_target and n < 7:
+    while t < seconds_target and n < cap - 2:
         t0 = time.perf_counter()
         o.step(1)
         t += time.perf_counter() - t0
