@@ -170,6 +170,58 @@ def cpu_baseline(m, seconds_target=15.0):
                       "%d steps (%.1f s) incl. per-step FP64 energy" % (m, m, n, t)}
 
 
+def reference_cpu_sample(timeout_s=150.0):
+    """The reference's own single-core acoustic solver, built from its sources
+    by oracle/ref_build.sh (oracle/_ref/singlecore_acoustic_f32: -O3, NDEBUG,
+    no FMA contraction), run on its hard-wired 600x600 periodic FP32 case until
+    it reports the time of its first 1000 steps; None if it was not built."""
+    import pty
+    import select
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "singlecore_acoustic_f32")
+    if not os.access(exe, os.X_OK):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        work = os.path.join(d, "a")  # the binary writes its results to ../result
+        os.makedirs(work)
+        # a pseudo-terminal keeps the child's stdio line-buffered
+        master, slave = pty.openpty()
+        p = subprocess.Popen([exe], cwd=work, stdout=slave, stderr=slave, stdin=subprocess.DEVNULL)
+        os.close(slave)
+        t0, secs, buf = time.perf_counter(), None, ""
+        try:
+            while secs is None and time.perf_counter() - t0 < timeout_s:
+                r, _, _ = select.select([master], [], [], 1.0)
+                if not r:
+                    if p.poll() is not None:
+                        break
+                    continue
+                try:
+                    chunk = os.read(master, 4096).decode(errors="replace")
+                except OSError:
+                    break
+                buf += chunk
+                lines = buf.split("\n")
+                buf = lines[-1]
+                for line in lines[:-1]:
+                    # printed every 1000 steps; the first one (step 0) times nothing
+                    if "time for 1000 time steps" in line:
+                        v = float(line.split(":")[-1])
+                        if v > 0:
+                            secs = v
+                            break
+        finally:
+            p.kill()
+            p.wait()
+            os.close(master)
+    if secs is None:
+        return None
+    return {"value": 600 * 600 * 1000 / secs, "unit": "grid-point updates/s", "cores": 1, "kind": "reference",
+            "sample": "reference singlecore acoustic solver (singlecore/src/main_cpp/cpu_acoustic.cpp, built by "
+                      "oracle/ref_build.sh), its 600x600 periodic FP32 case, first 1000 steps (%.1f s)" % secs}
+
+
 def c4_batch(args):
     """Batched acoustic shots: 64 problems of n x n cells, n uniform in
     [512, 2048] (seeded), dx = 10 m, each with its own smooth random velocity
@@ -328,6 +380,9 @@ def main():
     s.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_size or m)
+        ref = reference_cpu_sample()
+        if ref is not None:
+            out["cpu_baseline"]["reference_solver"] = ref
     print(json.dumps(out))
 
 
