@@ -244,7 +244,7 @@ def c4_batch(args):
         rho, vp, _ = W.smooth_acoustic_model(n, dx, seed=1000 + i)
         cfg = T.config(physics=T.ACOUSTIC, mx=n, my=n, dx=dx, num_dx=1, den_dx=100,
                        dt=0.5 * dx / (math.sqrt(2.0) * vmax), bc="CCCC", precision=T.PRECISION[args.precision],
-                       cpst="0", coef16=0, nt_max=K + 64, record_energy=0, stencil_form=1)
+                       cpst="0", coef16=args.coef16, nt_max=K + 64, record_energy=0, stencil_form=1)
         s = T.Solver(cfg)
         s.set_model(rho, vp, vp, 1, 100)
         f0 = float(rng.uniform(10.0, 25.0))
@@ -257,12 +257,14 @@ def c4_batch(args):
     ms = B.elapsed_ms()
     cells = sum(n * n for n in sizes)
     value = cells * K / (ms / 1e3)
-    Bu = 2 * 3 * {0: 8, 1: 4, 2: 2, 3: 2}[T.PRECISION[args.precision]] + 3 * 4
+    prec = T.PRECISION[args.precision]
+    Bu = algorithmic_bytes(prec, "0", args.coef16, physics=1)
     out = {"metric": "grid-pt updates/s (batched acoustic shots)", "value": value, "unit": "grid-point updates/s",
            "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "64 acoustic shots, n x n with n uniform in [512, 2048], dx 10 m, smooth random "
-                                  "velocity 1.5-4.5 km/s, Ricker 10-25 Hz, CCCC SAT, storage %s" % args.precision,
+                                  "velocity 1.5-4.5 km/s, Ricker 10-25 Hz, CCCC SAT, storage %s, coef16 %d" %
+                                  (args.precision, args.coef16),
                       "problems": nprob, "cells": cells, "tile_rows": rows},
            "roofline": {"bound": "hbm", "achieved": value * Bu / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": value * Bu / 1e9 / HBM_PEAK_GBS, "traffic": None, "bytes_per_update": Bu,

@@ -52,6 +52,9 @@ def test_config_default_and_check():
         T.check_config(T.config(precision="bf16", cpst="6R"))
     with pytest.raises(T.TwError):
         T.check_config(T.config(energy_form=1))    # acoustic periodic only
+    with pytest.raises(T.TwError) as e:
+        T.check_config(T.config(mx=140000, my=140000))   # over the 2^34-cell guard
+    assert e.value.code == 3                    # TW_ENOMEM, before any allocation
 
 
 def test_field_shapes():

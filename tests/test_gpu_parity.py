@@ -19,16 +19,21 @@ pytestmark = pytest.mark.gpu
 # 5-row chunk (many chunk boundaries and window warm-ups), the per-point
 # kernels, and the fused single-pass step (elastic, SBP on both axes; other
 # configurations fall back to the streaming kernels) with both chunk sizes.
+# The streaming kernels run the SBP band tiles as extra blocks of their core
+# launches; "stream-sepbands" keeps the bands in launches of their own.
 # Read by tw_create from the environment.
-KERNELS = [("2", "16"), ("2", "5"), ("1", "16"), ("3", "16"), ("3", "5")]
+KERNELS = [("2", "16", "1"), ("2", "5", "1"), ("2", "16", "0"), ("1", "16", "1"), ("3", "16", "1"), ("3", "5", "1")]
+ENV = ("TW_KERNEL", "TW_TXC", "TW_MERGE_BANDS")
 
 
-@pytest.fixture(params=KERNELS, ids=["stream", "stream-txc5", "point", "fused", "fused-txc5"], autouse=True)
+@pytest.fixture(params=KERNELS, ids=["stream", "stream-txc5", "stream-sepbands", "point", "fused", "fused-txc5"],
+                autouse=True)
 def kernel_variant(request):
-    old = (os.environ.get("TW_KERNEL"), os.environ.get("TW_TXC"))
-    os.environ["TW_KERNEL"], os.environ["TW_TXC"] = request.param
+    old = tuple(os.environ.get(k) for k in ENV)
+    for k, v in zip(ENV, request.param):
+        os.environ[k] = v
     yield request.param
-    for k, v in zip(("TW_KERNEL", "TW_TXC"), old):
+    for k, v in zip(ENV, old):
         if v is None:
             os.environ.pop(k, None)
         else:

@@ -78,6 +78,7 @@ struct Plan {
   void* dstat_ip[2] = {}; // in-place arguments for each parity
   void* dstat_pp[2] = {}; // fused-step arguments (read current, write other) for each parity
   int32_t txc = 16;   // rows per chunk of the streaming kernels (TW_TXC overrides)
+  int32_t merge_bands = 1;  // band tiles as extra blocks of the core launches (TW_MERGE_BANDS=0: own launch)
   // timing of this step's core launches: start/stop of velocity, stress (null: off)
   hipEvent_t prof[4] = {};
   double dx;

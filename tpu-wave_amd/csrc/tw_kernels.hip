@@ -523,12 +523,9 @@ __device__ __forceinline__ bool map_point(const Rects& R, int bx, int by, int* i
 }
 
 // Velocity half step: vx (NM) and vy (MN).
+// one point of the velocity half step (the body of the per-point kernels); returns its energy term
 template <class ST, class KT, class CT>
-__device__ __forceinline__ void vel_pt(const Stat<ST, KT, CT>& A, const Dyn& D, const Rects& R, int bx, int by,
-                                       double* sh) {
-  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
-  int ix, iy, pidx;
-  const bool inr = map_point(R, bx, by, &ix, &iy, &pidx);
+__device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn& D, int ix, int iy, bool inr) {
   const int el = A.physics == TW_ELASTIC;
   const int kmode = A.cpst == TW_CPST_K;
   double e = 0.0;
@@ -586,6 +583,16 @@ __device__ __forceinline__ void vel_pt(const Stat<ST, KT, CT>& A, const Dyn& D, 
       if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 0, 1, ix, iy, X.rows, X.cols);
     }
   }
+  return e;
+}
+
+template <class ST, class KT, class CT>
+__device__ __forceinline__ void vel_pt(const Stat<ST, KT, CT>& A, const Dyn& D, const Rects& R, int bx, int by,
+                                       double* sh) {
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
+  int ix, iy, pidx;
+  const bool inr = map_point(R, bx, by, &ix, &iy, &pidx);
+  const double e = vel_pt_e<ST, KT, CT>(A, D, ix, iy, inr);
   if (D.energy) {
     const double bsum = block_sum(e, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) D.part[pidx] = bsum;
@@ -599,12 +606,9 @@ __global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
 }
 
 // Stress half step: sxy (NN) and sxx, syy (MM), or p (MM, acoustic).
+// one point of the stress half step (the body of the per-point kernels); returns its energy term
 template <class ST, class KT, class CT>
-__device__ __forceinline__ void str_pt(const Stat<ST, KT, CT>& A, const Dyn& D, const Rects& R, int bx, int by,
-                                       double* sh) {
-  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
-  int ix, iy, pidx;
-  const bool inr = map_point(R, bx, by, &ix, &iy, &pidx);
+__device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn& D, int ix, int iy, bool inr) {
   const int kmode = A.cpst == TW_CPST_K;
   const int rmode = kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R;
   double e = 0.0;
@@ -700,6 +704,16 @@ __device__ __forceinline__ void str_pt(const Stat<ST, KT, CT>& A, const Dyn& D, 
       }
     }
   }
+  return e;
+}
+
+template <class ST, class KT, class CT>
+__device__ __forceinline__ void str_pt(const Stat<ST, KT, CT>& A, const Dyn& D, const Rects& R, int bx, int by,
+                                       double* sh) {
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
+  int ix, iy, pidx;
+  const bool inr = map_point(R, bx, by, &ix, &iy, &pidx);
+  const double e = str_pt_e<ST, KT, CT>(A, D, ix, iy, inr);
   if (D.energy) {
     const double bsum = block_sum(e, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) D.part[pidx] = bsum;
@@ -710,6 +724,36 @@ template <class ST, class KT, class CT>
 __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
   __shared__ double sh[8];
   str_pt<ST, KT, CT>(A, A, R, blockIdx.x, blockIdx.y, sh);
+}
+
+// A band tile (256 points) run by one 64-lane block of a core launch: four
+// points per lane, in the tile order of map_point; the wave's energy sum goes
+// to the tile's partial.  No pending work here: block 0 of the launch is a
+// core block.
+__device__ __forceinline__ bool map_tile_point(const Rects& R, int b, int lin, int* ix, int* iy) {
+  int k = 0;
+  while (k + 1 < R.n && b >= R.cum[k + 1]) k++;
+  const int t = b - R.cum[k];
+  const int tw = R.tw[k], th = 256 / tw;
+  const int tc = (R.c1[k] - R.c0[k] + tw - 1) / tw;
+  *ix = R.r0[k] + (t / tc) * th + lin / tw;
+  *iy = R.c0[k] + (t % tc) * tw + lin % tw;
+  return *ix < R.r1[k] && *iy < R.c1[k];
+}
+
+template <class ST, class KT, class CT, int HALF>
+__device__ __forceinline__ void band_tile(const Stat<ST, KT, CT>& A, const Dyn& D, const Rects& R, int b) {
+  double e = 0.0;
+#pragma unroll 1
+  for (int sub = 0; sub < 4; sub++) {
+    int ix, iy;
+    const bool inr = map_tile_point(R, b, sub * 64 + (int)threadIdx.x, &ix, &iy);
+    e += HALF == 0 ? vel_pt_e<ST, KT, CT>(A, D, ix, iy, inr) : str_pt_e<ST, KT, CT>(A, D, ix, iy, inr);
+  }
+  if (D.energy) {
+    const double t = wave_sum(e);
+    if (threadIdx.x == 0) D.part[R.poff + b] = t;
+  }
 }
 
 // Energy of the stress fields only (start of a run / after the state was set).
@@ -1592,8 +1636,14 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
 
 template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
-k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
-  vel3_body<ST, KT, CT, PH, CP>(Sp, D, C, blockIdx.x, blockIdx.y, gridDim.x);
+k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C, int nbx, int ncore, Rects R) {
+  // blocks [0, ncore): core tiles; after them the band tiles of this half step
+  const int L = blockIdx.x;
+  if (L >= ncore) {
+    band_tile<ST, KT, CT, 0>(*Sp, D, R, L - ncore);
+    return;
+  }
+  vel3_body<ST, KT, CT, PH, CP>(Sp, D, C, L % nbx, L / nbx, nbx);
 }
 
 // Stress half step, core region, streaming.
@@ -1942,8 +1992,14 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
 
 template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
-k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C) {
-  str3_body<ST, KT, CT, PH, CP>(Sp, D, C, blockIdx.x, blockIdx.y, gridDim.x);
+k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C, int nbx, int ncore, Rects R) {
+  // blocks [0, ncore): core tiles; after them the band tiles of this half step
+  const int L = blockIdx.x;
+  if (L >= ncore) {
+    band_tile<ST, KT, CT, 1>(*Sp, D, R, L - ncore);
+    return;
+  }
+  str3_body<ST, KT, CT, PH, CP>(Sp, D, C, L % nbx, L / nbx, nbx);
 }
 
 // ------------------------------------------------------------ dispatch
@@ -2016,38 +2072,40 @@ int energy_parts(const Plan& p) {
 
 template <class ST, class KT, class CT, int PH, int CP>
 static void launch_core_t(int half, const Stat<ST, KT, CT>* sp, const Dyn& d, const Core& C, dim3 g, dim3 b,
-                          hipStream_t s) {
+                          hipStream_t s, const Rects& R, int nband) {
+  const int ncore = (int)(g.x * g.y);
+  const dim3 g1((unsigned)(ncore + nband));
   if (half == 0)
-    hipLaunchKernelGGL((k_vel3<ST, KT, CT, PH, CP>), g, b, 0, s, sp, d, C);
+    hipLaunchKernelGGL((k_vel3<ST, KT, CT, PH, CP>), g1, b, 0, s, sp, d, C, (int)g.x, ncore, R);
   else
-    hipLaunchKernelGGL((k_str3<ST, KT, CT, PH, CP>), g, b, 0, s, sp, d, C);
+    hipLaunchKernelGGL((k_str3<ST, KT, CT, PH, CP>), g1, b, 0, s, sp, d, C, (int)g.x, ncore, R);
 }
 
 template <class ST, class KT, class CT, int PH>
 static hipError_t launch_core_p(const Plan& p, int half, const Stat<ST, KT, CT>* sp, const Dyn& d, const Core& C,
-                                dim3 g, dim3 b, hipStream_t s) {
+                                dim3 g, dim3 b, hipStream_t s, const Rects& R, int nband) {
   if constexpr (sizeof(ST) >= 4) {
     switch (p.cpst) {
-      case TW_CPST_0: launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s); break;
-      case TW_CPST_3R: launch_core_t<ST, KT, CT, PH, TW_CPST_3R>(half, sp, d, C, g, b, s); break;
-      case TW_CPST_M3R: launch_core_t<ST, KT, CT, PH, TW_CPST_M3R>(half, sp, d, C, g, b, s); break;
-      case TW_CPST_6R: launch_core_t<ST, KT, CT, PH, TW_CPST_6R>(half, sp, d, C, g, b, s); break;
+      case TW_CPST_0: launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s, R, nband); break;
+      case TW_CPST_3R: launch_core_t<ST, KT, CT, PH, TW_CPST_3R>(half, sp, d, C, g, b, s, R, nband); break;
+      case TW_CPST_M3R: launch_core_t<ST, KT, CT, PH, TW_CPST_M3R>(half, sp, d, C, g, b, s, R, nband); break;
+      case TW_CPST_6R: launch_core_t<ST, KT, CT, PH, TW_CPST_6R>(half, sp, d, C, g, b, s, R, nband); break;
       default: return hipErrorInvalidValue;
     }
   } else {
     if (p.cpst == TW_CPST_K)
-      launch_core_t<ST, KT, CT, PH, TW_CPST_K>(half, sp, d, C, g, b, s);
+      launch_core_t<ST, KT, CT, PH, TW_CPST_K>(half, sp, d, C, g, b, s, R, nband);
     else
-      launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s);
+      launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s, R, nband);
   }
   return hipGetLastError();
 }
 
 template <class ST, class KT, class CT>
 static hipError_t launch_core(const Plan& p, int half, const Stat<ST, KT, CT>* sp, const Dyn& d, const Core& C, dim3 g,
-                              dim3 b, hipStream_t s) {
-  if (p.physics == TW_ELASTIC) return launch_core_p<ST, KT, CT, TW_ELASTIC>(p, half, sp, d, C, g, b, s);
-  return launch_core_p<ST, KT, CT, TW_ACOUSTIC>(p, half, sp, d, C, g, b, s);
+                              dim3 b, hipStream_t s, const Rects& R, int nband) {
+  if (p.physics == TW_ELASTIC) return launch_core_p<ST, KT, CT, TW_ELASTIC>(p, half, sp, d, C, g, b, s, R, nband);
+  return launch_core_p<ST, KT, CT, TW_ACOUSTIC>(p, half, sp, d, C, g, b, s, R, nband);
 }
 
 template <class ST, class KT, class CT>
@@ -2071,16 +2129,21 @@ static hipError_t run_half(const Plan& p, int half, const StepCtl& c, hipStream_
   grid_dims(p, &g, &b);
   const Stat<ST, KT, CT>* sp = (const Stat<ST, KT, CT>*)p.dstat;
   const Dyn d = a;
-  if (C.rhi > C.rlo && C.chi > C.clo) {
+  // the band tiles run as extra blocks of the core launch (p.merge_bands), or
+  // as a launch of their own after it
+  const bool core = C.rhi > C.rlo && C.chi > C.clo;
+  const bool merged = core && p.merge_bands && R.n > 0;
+  if (core) {
+    R.poff = (int32_t)(g.x * g.y);
     if (p.prof[2 * half]) (void)hipEventRecord(p.prof[2 * half], s);
-    hipError_t e = launch_core<ST, KT, CT>(p, half, sp, d, C, g, b, s);
+    hipError_t e = launch_core<ST, KT, CT>(p, half, sp, d, C, g, b, s, R, merged ? R.cum[R.n] : 0);
     if (e != hipSuccess) return e;
     if (p.prof[2 * half + 1]) (void)hipEventRecord(p.prof[2 * half + 1], s);
   } else {
     // no core rows: the band launch carries the pending work
     R.poff = 0;
   }
-  if (R.n > 0) {
+  if (R.n > 0 && !merged) {
     R.poff = (int32_t)(g.x * g.y);
     Args<ST, KT, CT> ab = a;
     if (C.rhi > C.rlo && C.chi > C.clo) ab.pend = Pending();  // done by the core launch
@@ -2611,7 +2674,15 @@ __device__ __forceinline__ int batch_item(const int32_t* b0, int stride_words, i
 
 template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
-k_vel3b(const BatchCore* __restrict__ items, int n) {
+k_vel3b(const BatchCore* __restrict__ items, int n, const BatchBand* __restrict__ bands, int nband,
+        int core_blocks) {
+  // blocks [0, core_blocks): core tiles of all problems; after them their band tiles
+  if ((int)blockIdx.x >= core_blocks) {
+    const int bb = blockIdx.x - core_blocks;
+    const BatchBand& E = bands[batch_item(&bands[0].b0, sizeof(BatchBand) / 4, nband, bb)];
+    band_tile<ST, KT, CT, 0>(*(const Stat<ST, KT, CT>*)E.sp, E.d, E.r, bb - E.b0);
+    return;
+  }
   const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, blockIdx.x);
   const BatchCore& B = items[k];
   const int l = blockIdx.x - B.b0;
@@ -2620,7 +2691,15 @@ k_vel3b(const BatchCore* __restrict__ items, int n) {
 
 template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP>)))
-k_str3b(const BatchCore* __restrict__ items, int n) {
+k_str3b(const BatchCore* __restrict__ items, int n, const BatchBand* __restrict__ bands, int nband,
+        int core_blocks) {
+  // blocks [0, core_blocks): core tiles of all problems; after them their band tiles
+  if ((int)blockIdx.x >= core_blocks) {
+    const int bb = blockIdx.x - core_blocks;
+    const BatchBand& E = bands[batch_item(&bands[0].b0, sizeof(BatchBand) / 4, nband, bb)];
+    band_tile<ST, KT, CT, 1>(*(const Stat<ST, KT, CT>*)E.sp, E.d, E.r, bb - E.b0);
+    return;
+  }
   const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, blockIdx.x);
   const BatchCore& B = items[k];
   const int l = blockIdx.x - B.b0;
@@ -2640,13 +2719,15 @@ __global__ void __launch_bounds__(256) k_bandb(const BatchBand* __restrict__ ite
 }
 
 template <class ST, class KT, class CT, int PH>
-static void launch_batch_core(const Plan& p, int half, const BatchCore* items, int n, int blocks, hipStream_t s) {
+static void launch_batch_core(const Plan& p, int half, const BatchCore* items, int n, int blocks,
+                              const BatchBand* bands, int nband, int band_blocks, hipStream_t s) {
+  const dim3 g((unsigned)(blocks + band_blocks));
   auto go = [&](auto cp) {
     constexpr int CP = decltype(cp)::value;
     if (half == 0)
-      hipLaunchKernelGGL((k_vel3b<ST, KT, CT, PH, CP>), dim3(blocks), dim3(64), 0, s, items, n);
+      hipLaunchKernelGGL((k_vel3b<ST, KT, CT, PH, CP>), g, dim3(64), 0, s, items, n, bands, nband, blocks);
     else
-      hipLaunchKernelGGL((k_str3b<ST, KT, CT, PH, CP>), dim3(blocks), dim3(64), 0, s, items, n);
+      hipLaunchKernelGGL((k_str3b<ST, KT, CT, PH, CP>), g, dim3(64), 0, s, items, n, bands, nband, blocks);
   };
   if constexpr (sizeof(ST) >= 4) {
     switch (p.cpst) {
@@ -2666,13 +2747,17 @@ static void launch_batch_core(const Plan& p, int half, const BatchCore* items, i
 template <class ST, class KT, class CT>
 static hipError_t run_batch_half(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
                                  const BatchBand* band, int nband, int band_blocks, hipStream_t s) {
-  if (ncore > 0) {
+  // the band tiles run as extra blocks of the core launch when there is one
+  const bool merged = ncore > 0 && core_blocks > 0 && p.merge_bands;
+  if (ncore > 0 && core_blocks > 0) {
+    const int bb = merged ? band_blocks : 0;
     if (p.physics == TW_ELASTIC)
-      launch_batch_core<ST, KT, CT, TW_ELASTIC>(p, half, core, ncore, core_blocks, s);
+      launch_batch_core<ST, KT, CT, TW_ELASTIC>(p, half, core, ncore, core_blocks, band, nband, bb, s);
     else
-      launch_batch_core<ST, KT, CT, TW_ACOUSTIC>(p, half, core, ncore, core_blocks, s);
+      launch_batch_core<ST, KT, CT, TW_ACOUSTIC>(p, half, core, ncore, core_blocks, band, nband, bb, s);
   }
-  if (nband > 0) hipLaunchKernelGGL((k_bandb<ST, KT, CT>), dim3(band_blocks), dim3(64, 4), 0, s, band, nband, half);
+  if (nband > 0 && band_blocks > 0 && !merged)
+    hipLaunchKernelGGL((k_bandb<ST, KT, CT>), dim3(band_blocks), dim3(64, 4), 0, s, band, nband, half);
   return hipGetLastError();
 }
 
