@@ -114,6 +114,42 @@ def test_acoustic_reference_energy_form():
         np.testing.assert_allclose(a, b, rtol=ENERGY_RTOL)
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("prec,name", [("fp64", "f64"), ("fp32", "f32")])
+def test_gpu_against_reference_golden(prec, name):
+    """The GPU solver on the reference's own single-core acoustic case
+    (singlecore/src/main_cpp/cpu_acoustic.cpp: 600^2 periodic, dx = 0.008,
+    dt = 1e-4, Ricker f = 5, t0 = 0.25 on P at (200, 200), receivers at
+    (400, 400)), against the reference program's printed output
+    (tests/golden, "% 16.15e"): receiver traces digit for digit over the
+    first 20000 steps (the wave arrives after ~17000), energies
+    E = E_Vx + E_Vy + (E_P(n) + E_P(n-1))/2 to 1e-12 (the summation order
+    differs)."""
+    g = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s.npz" % name))
+    nt = 20000
+    cfg = T.config(physics=T.ACOUSTIC, mx=600, my=600, dx=(1.0 / (5.0 * 2.5)) / 10, num_dx=0, den_dx=0, dt=1e-4,
+                   bc="PPPP", precision=prec, cpst="0", stencil_form=1, energy_form=1, nt_max=nt)
+    s = T.Solver(cfg)
+    s.set_source_ricker(T.GRID_MM, 200, 200, 5.0, 0.25)
+    s.set_receivers([T.VX, T.VY, T.P], [400, 400, 400], [400, 400, 400])
+    s.step(nt)
+    ek, ep0, ep1 = s.energy()
+    E = (ek + ep1 / 2.0) + ep0 / 2.0
+    np.testing.assert_allclose(E[:3000], g["E_head"], rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(E[::50][: nt // 50], g["E_every"][: nt // 50], rtol=1e-12, atol=1e-300)
+    rec = s.receivers()
+    it = g["trace_it"]
+    it = it[it < nt]
+    assert len(it) > 100
+    fmt = lambda v: np.array([float("% 16.15e" % x) for x in v])
+    for k, f in enumerate(("Vx", "Vy", "P")):
+        np.testing.assert_array_equal(fmt(rec[k][it]), g[f][: len(it)], err_msg=f)
+    assert np.abs(rec[2][it]).max() > 0
+    s.close()
+
+
 def test_model_interpolation_parity():
     """tw_set_model (host) and the oracle's restatement give identical coefficients."""
     s, o = C.make_pair(physics="elastic", mx=40, my=48, bc="FFPP", precision="fp32", nt=10, num_dx=1, den_dx=100)
