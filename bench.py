@@ -4,9 +4,10 @@ Workload (the mixed-precision elastic configuration): 8192x8192 cells,
 dx = 5 m, seeded 20-layer model (Vp 1.5-4 km/s per layer, 5% correlated
 perturbation, Vs = Vp/1.73, rho = 310 Vp^0.25), 15 Hz Ricker vertical force at
 100 m depth, free-surface SAT on top, characteristic SAT on the other sides,
-256 surface receivers.  Fields stored in BF16 with a BF16 Kahan residual
-(compensated time-stage sums), FP32 arithmetic, material coefficients stored
-in BF16; the FP32 and FP64 comparison runs use the same quantized material.
+256 surface receivers.  Fields stored in BF16 with the Kahan residual of each
+store kept as an 8-bit code (compensated time-stage sums), FP32 arithmetic,
+material coefficients stored in BF16; the FP32 and FP64 comparison runs use
+the same quantized material.
 
 A step is one full leapfrog step (velocity and stress half steps, SAT,
 compensated update, source, receivers, FP64 energy) of all 5 fields.
@@ -33,13 +34,20 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "grid-pt updates/s & % HBM peak; BF16-store vs FP32 at ≤1e-4 rel L2 vs FP64"
 
 
+def comp_bytes(s, cpst):
+    """Bytes of the stored compensation per value: a storage-type companion
+    array, one byte for the K8 residual code, none for cpst 0."""
+    if cpst == "K8":
+        return 1
+    return s if cpst in ("3R", "-3R", "6R", "3C", "-3C", "6C", "K") else 0
+
+
 def algorithmic_bytes(precision, cpst, coef16, physics=0):
     """Compulsory HBM bytes per cell update: 2 F s (state r+w) + 2 F c
     (stored compensation r+w) + K p (coefficients read)."""
     F, K = (5, 5) if physics == 0 else (3, 3)
     s = {0: 8, 1: 4, 2: 2, 3: 2}[precision]
-    comp = cpst in ("3R", "-3R", "6R", "3C", "-3C", "6C", "K")
-    c = s if comp else 0
+    c = comp_bytes(s, cpst)
     p = 8 if precision == 0 else (2 if coef16 else 4)
     return 2 * F * s + 2 * F * c + K * p
 
@@ -52,8 +60,7 @@ def half_bytes(precision, cpst, coef16, half, physics=0):
     halves sum to algorithmic_bytes."""
     F = 2 if half == 0 else (3 if physics == 0 else 1)
     s = {0: 8, 1: 4, 2: 2, 3: 2}[precision]
-    comp = cpst in ("3R", "-3R", "6R", "3C", "-3C", "6C", "K")
-    c = s if comp else 0
+    c = comp_bytes(s, cpst)
     p = 8 if precision == 0 else (2 if coef16 else 4)
     return 2 * F * s + 2 * F * c + F * p
 
@@ -279,7 +286,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--precision", default="bf16")
-    ap.add_argument("--cpst", default="K", help="compensation; K = 16-bit Kahan residual (the mixed-precision config)")
+    ap.add_argument("--cpst", default="K8",
+                    help="compensation; K8 = the Kahan residual kept as an 8-bit code per value (the mixed-precision "
+                    "config), K = 16-bit residual, 0 = none")
     ap.add_argument("--coef16", type=int, default=1, help="store material coefficients in the 16-bit type")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32 and FP64 runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

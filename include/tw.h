@@ -58,10 +58,12 @@ enum tw_precision { TW_FP64 = 0, TW_FP32 = 1, TW_BF16 = 2, TW_FP16 = 3 };
  *   TW_CPST_M3C explicit 3-flop form of the above  (-3,'C')
  *   TW_CPST_6C  2Sum(S,R+C) -> (S,C), R = 0        ( 6,'C')
  *   TW_CPST_K   16-bit storage only: FP32 sum, the rounding residual of the
- *               store is kept in a 16-bit companion array (Kahan-style) */
+ *               store is kept in a 16-bit companion array (Kahan-style)
+ *   TW_CPST_K8  as K with the residual kept at 8 bits: a code q in
+ *               [-127, 127] per value, residual = q * ulp(stored value) / 256 */
 enum tw_cpst {
   TW_CPST_0 = 0, TW_CPST_3R = 1, TW_CPST_M3R = 2, TW_CPST_6R = 3,
-  TW_CPST_3C = 4, TW_CPST_M3C = 5, TW_CPST_6C = 6, TW_CPST_K = 7
+  TW_CPST_3C = 4, TW_CPST_M3C = 5, TW_CPST_6C = 6, TW_CPST_K = 7, TW_CPST_K8 = 8
 };
 enum tw_field { TW_VX = 0, TW_VY = 1, TW_SXX = 2, TW_SYY = 3, TW_SXY = 4, TW_P = 5 };
 enum tw_grid  { TW_GRID_NN = 0, TW_GRID_MN = 1, TW_GRID_NM = 2, TW_GRID_MM = 3 };

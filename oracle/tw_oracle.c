@@ -206,11 +206,23 @@ static double or_hweight_f(const tw_config* c, int axis, int t, int i, int n, co
 #undef C2S
 #undef REFMODE
 
+/* exponent field of a 16-bit stored value and the bias that makes
+ * 2^(max(e, 1) - bias) = ulp / 256 (the unit of the 8-bit residual code):
+ * BF16 ulp = 2^(e - 127 - 7), FP16 ulp = 2^(e - 15 - 10) */
+static inline int bf16_exp(uint16_t h) { return (h >> 7) & 0xff; }
+static inline int f16_exp(_Float16 h) {
+  uint16_t b;
+  memcpy(&b, &h, 2);
+  return (b >> 10) & 0x1f;
+}
+
 #define PFX bf16
 #define SREAL uint16_t
 #define CREAL float
 #define S2C(x) bf16_to_f32(x)
 #define C2S(x) f32_to_bf16((float)(x))
+#define S_EXP(x) bf16_exp(x)
+#define K8_BIAS 142
 #define REFMODE 0
 #include "tw_oracle_body.h"
 #undef PFX
@@ -218,6 +230,8 @@ static double or_hweight_f(const tw_config* c, int axis, int t, int i, int n, co
 #undef CREAL
 #undef S2C
 #undef C2S
+#undef S_EXP
+#undef K8_BIAS
 #undef REFMODE
 
 #define PFX f16
@@ -225,6 +239,8 @@ static double or_hweight_f(const tw_config* c, int axis, int t, int i, int n, co
 #define CREAL float
 #define S2C(x) ((float)(x))
 #define C2S(x) ((_Float16)(float)(x))
+#define S_EXP(x) f16_exp(x)
+#define K8_BIAS 33
 #define REFMODE 0
 #include "tw_oracle_body.h"
 #undef PFX
@@ -232,6 +248,8 @@ static double or_hweight_f(const tw_config* c, int axis, int t, int i, int n, co
 #undef CREAL
 #undef S2C
 #undef C2S
+#undef S_EXP
+#undef K8_BIAS
 #undef REFMODE
 #undef or_hweight
 
@@ -246,10 +264,13 @@ static int cfg_check(const tw_config* c) {
   for (int a = 0; a < 2; a++)
     if ((c->bc[2 * a] == 'P') != (c->bc[2 * a + 1] == 'P')) { or_seterr("periodic must pair"); return TW_EINVAL; }
   if (c->precision < TW_FP64 || c->precision > TW_FP16) { or_seterr("precision"); return TW_EINVAL; }
-  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K) { or_seterr("cpst"); return TW_EINVAL; }
+  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K8) { or_seterr("cpst"); return TW_EINVAL; }
   int ref = c->precision == TW_FP64 || c->precision == TW_FP32;
-  if (ref && c->cpst == TW_CPST_K) { or_seterr("cpst K needs 16-bit storage"); return TW_EINVAL; }
-  if (!ref && c->cpst != TW_CPST_0 && c->cpst != TW_CPST_K) { or_seterr("16-bit storage supports cpst 0 or K"); return TW_EINVAL; }
+  if (ref && (c->cpst == TW_CPST_K || c->cpst == TW_CPST_K8)) { or_seterr("cpst K/K8 needs 16-bit storage"); return TW_EINVAL; }
+  if (!ref && c->cpst != TW_CPST_0 && c->cpst != TW_CPST_K && c->cpst != TW_CPST_K8) {
+    or_seterr("16-bit storage supports cpst 0, K or K8");
+    return TW_EINVAL;
+  }
   if (c->energy_form == 1 && (c->physics != TW_ACOUSTIC || c->bc[0] != 'P' || c->bc[2] != 'P')) {
     or_seterr("energy_form 1 is the periodic acoustic reference form");
     return TW_EINVAL;

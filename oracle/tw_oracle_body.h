@@ -192,10 +192,28 @@ static void FN(get_field)(void* p, int f, double* out, int which) {
   for (size_t i = 0; i < n; i++) out[i] = d ? (double)S2C(d[i]) : 0.0;
 }
 
+#if !REFMODE
+/* K8: the residual array holds an integer code q in [-127, 127] (exact in the
+ * storage type); the residual is q * ulp(hi) / 256 of the stored value hi */
+static inline CREAL FN(k8_dec)(SREAL q, SREAL hi) {
+  const int e = S_EXP(hi);
+  return ldexpf(S2C(q), (e > 0 ? e : 1) - K8_BIAS);
+}
+static inline SREAL FN(k8_enc)(CREAL x, SREAL hi) {
+  const int e = S_EXP(hi);
+  float q = rintf(ldexpf(x, K8_BIAS - (e > 0 ? e : 1)));
+  q = q > 127.f ? 127.f : (q < -127.f ? -127.f : q);
+  return C2S(q);
+}
+#endif
+
 /* value of a stored point as seen by receivers and energy: the state, plus the
- * Kahan residual in the 16-bit K variant */
+ * Kahan residual in the 16-bit K (kmode 1) and K8 (kmode 2) variants */
 static inline CREAL FN(val)(const FN(fld) * F, size_t i, int kmode) {
   CREAL v = S2C(F->s[i]);
+#if !REFMODE
+  if (kmode == 2) return v + FN(k8_dec)(F->r[i], F->s[i]);
+#endif
   if (kmode) v = v + S2C(F->r[i]);
   return v;
 }
@@ -210,8 +228,9 @@ static void FN(acc_init)(FN(st_t) * S, int f) {
 #if REFMODE
   for (size_t i = 0; i < n; i++) F->acc[i] = F->r[i];
 #else
-  int k = S->cm->cfg.cpst == TW_CPST_K;
-  for (size_t i = 0; i < n; i++) F->acc[i] = k ? S2C(F->r[i]) : (CREAL)0;
+  int k = S->cm->cfg.cpst == TW_CPST_K, k8 = S->cm->cfg.cpst == TW_CPST_K8;
+  for (size_t i = 0; i < n; i++)
+    F->acc[i] = k ? S2C(F->r[i]) : (k8 ? FN(k8_dec)(F->r[i], F->s[i]) : (CREAL)0);
 #endif
 }
 
@@ -386,6 +405,7 @@ static void FN(update)(FN(st_t) * S, int f) {
     SREAL hi = C2S(t);
     F->s[i] = hi;
     if (cpst == TW_CPST_K) F->r[i] = C2S(t - S2C(hi));
+    if (cpst == TW_CPST_K8) F->r[i] = FN(k8_enc)(t - S2C(hi), hi);
 #endif
   }
 }
@@ -413,6 +433,12 @@ static void FN(apply_source)(FN(st_t) * S, int half) {
       SREAL hi = C2S(vf);
       F->s[q] = hi;
       F->r[q] = C2S(vf - S2C(hi));
+    } else if (cm->cfg.cpst == TW_CPST_K8) {
+      double v = (double)S2C(F->s[q]) + (double)FN(k8_dec)(F->r[q], F->s[q]) + inc;
+      CREAL vf = (CREAL)v;
+      SREAL hi = C2S(vf);
+      F->s[q] = hi;
+      F->r[q] = FN(k8_enc)(vf - S2C(hi), hi);
     } else {
       F->s[q] = C2S((CREAL)((double)S2C(F->s[q]) + inc));
     }
@@ -425,7 +451,7 @@ static double FN(energy_field)(FN(st_t) * S, int f) {
   or_common* cm = S->cm;
   const tw_config* c = &cm->cfg;
   FN(fld)* F = &S->f[f];
-  int kmode = (c->cpst == TW_CPST_K);
+  int kmode = c->cpst == TW_CPST_K ? 1 : (c->cpst == TW_CPST_K8 ? 2 : 0);
   int g = or_field_grid(c->physics, f);
   int tx = (g == TW_GRID_NN || g == TW_GRID_NM), ty = (g == TW_GRID_NN || g == TW_GRID_MN);
   double E = 0.0;
@@ -469,7 +495,7 @@ static double FN(energy_mm)(FN(st_t) * S) {
   const tw_config* c = &cm->cfg;
   FN(fld)* A = &S->f[TW_SXX];
   FN(fld)* B = &S->f[TW_SYY];
-  int kmode = (c->cpst == TW_CPST_K);
+  int kmode = c->cpst == TW_CPST_K ? 1 : (c->cpst == TW_CPST_K8 ? 2 : 0);
   const CREAL* lam = S->coef[TW_C_LAMBDA];
   const CREAL* tmu = S->coef[TW_C_TWOMU];
   double E = 0.0;
@@ -563,7 +589,7 @@ static void FN(one_step)(void* p) {
     cm->ep1[it] = cm->ep_prev;
   }
   if (it < c->nt_max) {
-    int kmode = c->cpst == TW_CPST_K;
+    int kmode = c->cpst == TW_CPST_K ? 1 : (c->cpst == TW_CPST_K8 ? 2 : 0);
     for (int r = 0; r < cm->nrcv; r++) {
       FN(fld)* F = &S->f[cm->rf[r]];
       size_t q = (size_t)cm->rix[r] * F->cols + cm->riy[r];

@@ -64,14 +64,20 @@ def receivers_all(solver, orc, seed, n_per_field=3):
     return fs, xs, ys
 
 
-def compare_state(solver, orc, exact=True, rtol=0.0):
+def compare_state(solver, orc, exact=True, rtol=0.0, companions=False):
+    """State arrays of every field; with companions=True also the
+    compensation companion (R or C array, the K residual, the K8 byte codes;
+    zeros on both sides where the variant keeps none)."""
     for f in solver.fields:
-        a = solver.get_field(f)
-        b = orc.get_field(f)
-        if exact:
-            np.testing.assert_array_equal(a, b, err_msg="field %s" % T.FIELD_NAMES[f])
-        else:
-            np.testing.assert_allclose(a, b, rtol=rtol, atol=0, err_msg="field %s" % T.FIELD_NAMES[f])
+        pairs = [("field", solver.get_field(f), orc.get_field(f))]
+        if companions:
+            pairs.append(("companion", solver.get_companion(f), orc.get_companion(f)))
+        for what, a, b in pairs:
+            msg = "%s %s" % (what, T.FIELD_NAMES[f])
+            if exact:
+                np.testing.assert_array_equal(a, b, err_msg=msg)
+            else:
+                np.testing.assert_allclose(a, b, rtol=rtol, atol=0, err_msg=msg)
 
 
 def rel_l2(a, b):

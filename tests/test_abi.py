@@ -46,8 +46,10 @@ def test_config_default_and_check():
             T.check_config(d)
     with pytest.raises(T.TwError):
         T.check_config(T.config(bc="PFPP"))        # periodic must pair
-    with pytest.raises(T.TwError):
-        T.check_config(T.config(precision="fp32", cpst="K"))
+    for cp in ("K", "K8"):
+        with pytest.raises(T.TwError):
+            T.check_config(T.config(precision="fp32", cpst=cp))
+        T.check_config(T.config(precision="bf16", cpst=cp))
     with pytest.raises(T.TwError):
         T.check_config(T.config(precision="bf16", cpst="6R"))
     with pytest.raises(T.TwError):

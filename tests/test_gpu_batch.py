@@ -41,9 +41,10 @@ def assert_same(a, b):
     for x, y in zip(a, b):
         for f in x.fields:
             np.testing.assert_array_equal(x.get_field(f), y.get_field(f), err_msg=T.FIELD_NAMES[f])
+            np.testing.assert_array_equal(x.get_companion(f), y.get_companion(f), err_msg=T.FIELD_NAMES[f])
 
 
-@pytest.mark.parametrize("precision,cpst", [("fp32", "0"), ("bf16", "K"), ("bf16", "0"), ("fp64", "6R")])
+@pytest.mark.parametrize("precision,cpst", [("fp32", "0"), ("bf16", "K"), ("bf16", "0"), ("fp64", "6R"), ("bf16", "K8")])
 def test_batch_equals_solo_acoustic(precision, cpst):
     print("case batch acoustic", precision, cpst, flush=True)
     batched, solo = pairs("acoustic", precision, cpst)

@@ -59,7 +59,7 @@ def run_case(physics, bc, precision, cpst="0", coef16=0, mx=40, my=48, nt=30, se
     s.step(nt // 2)
     s.step(nt - nt // 2)  # two calls: the carried stress work crosses a tw_step boundary
     o.step(nt)
-    C.compare_state(s, o, exact=True)
+    C.compare_state(s, o, exact=True, companions=True)
     np.testing.assert_array_equal(s.receivers(), o.receivers())
     es, eo = s.energy(), o.energy()
     for a, b, name in zip(es, eo, ("ek", "ep0", "ep1")):
@@ -74,7 +74,8 @@ def test_elastic_reference_modes(precision, cpst):
 
 
 @pytest.mark.parametrize("precision,cpst,coef16", [("bf16", "0", 0), ("bf16", "K", 0), ("bf16", "K", 1),
-                                                   ("fp16", "0", 0), ("fp16", "K", 0), ("fp16", "K", 1)])
+                                                   ("fp16", "0", 0), ("fp16", "K", 0), ("fp16", "K", 1),
+                                                   ("bf16", "K8", 0), ("bf16", "K8", 1), ("fp16", "K8", 1)])
 def test_elastic_16bit(precision, cpst, coef16):
     run_case("elastic", "FFPP", precision, cpst, coef16)
 
@@ -93,9 +94,16 @@ def test_elastic_wide(bc, precision):
 
 
 @pytest.mark.parametrize("bc", ["PPPP", "FFFF", "CCCC", "FCPP", "PPFC"])
-@pytest.mark.parametrize("precision,cpst", [("fp64", "0"), ("fp32", "3R"), ("bf16", "K"), ("fp16", "0")])
+@pytest.mark.parametrize("precision,cpst", [("fp64", "0"), ("fp32", "3R"), ("bf16", "K"), ("fp16", "0"),
+                                            ("fp16", "K8")])
 def test_acoustic(bc, precision, cpst):
     run_case("acoustic", bc, precision, cpst, stencil_form=1)
+
+
+@pytest.mark.parametrize("bc,precision", [("FCFC", "bf16"), ("PPPP", "fp16"), ("PPCC", "bf16")])
+def test_elastic_wide_k8(bc, precision):
+    """The 8-bit residual across strip halos and a ragged last strip."""
+    run_case("elastic", bc, precision, "K8", coef16=1, mx=21, my=531, nt=12)
 
 
 @pytest.mark.parametrize("bc", ["PPPP", "CCCC", "PPFC"])
@@ -169,8 +177,9 @@ def test_model_interpolation_parity():
     C.compare_state(s, o)
 
 
-def test_checkpoint_resume_bitwise():
-    s, _ = C.make_pair(physics="elastic", mx=40, my=48, bc="FCCC", precision="fp32", cpst="6R", nt=40)
+@pytest.mark.parametrize("precision,cpst", [("fp32", "6R"), ("bf16", "K8")])
+def test_checkpoint_resume_bitwise(precision, cpst):
+    s, _ = C.make_pair(physics="elastic", mx=40, my=48, bc="FCCC", precision=precision, cpst=cpst, nt=40)
     s.set_source_ricker(T.GRID_NM, 5, 20, 25.0, 0.01)
     s.step(15)
     blob = s.checkpoint()
@@ -181,6 +190,10 @@ def test_checkpoint_resume_bitwise():
     s.step(25)
     for f in s.fields:
         np.testing.assert_array_equal(s.get_field(f), ref[f])
+    s.reset()
+    assert s.current_step() == 0
+    for f in s.fields:
+        assert not s.get_field(f).any() and not s.get_companion(f).any()
 
 
 def test_determinism():

@@ -481,3 +481,4 @@ def test_compensation_reduces_error(physics):
     for prec, ratio in ((O.BF16, 0.1), (O.FP16, 0.2)):
         e16 = compensation_error(prec, "0", physics)
         assert compensation_error(prec, "K", physics) < ratio * e16
+        assert compensation_error(prec, "K8", physics) < ratio * e16

@@ -87,7 +87,7 @@ bool coef_in(int physics, int k) {
 }
 bool is_velocity(int f) { return f == TW_VX || f == TW_VY; }
 bool ref_precision(int p) { return p == TW_FP64 || p == TW_FP32; }
-bool rmode(int c) { return c == TW_CPST_3R || c == TW_CPST_M3R || c == TW_CPST_6R || c == TW_CPST_K; }
+bool rmode(int c) { return c == TW_CPST_3R || c == TW_CPST_M3R || c == TW_CPST_6R || c == TW_CPST_K || c == TW_CPST_K8; }
 bool cmode(int c) { return c == TW_CPST_3C || c == TW_CPST_M3C || c == TW_CPST_6C; }
 
 // ---- host conversions (round to nearest even) ----
@@ -106,6 +106,11 @@ float bf16_to_f32(uint16_t h) {
 }
 
 size_t state_size(int precision) { return precision == TW_FP64 ? 8 : (precision == TW_FP32 ? 4 : 2); }
+// element size of array `which` of a field (0 state, 1 R companion, 2 C
+// companion): the K8 residual holds one byte code per value
+size_t array_size(const tw_config* c, int which) {
+  return which == 1 && c->cpst == TW_CPST_K8 ? 1 : state_size(c->precision);
+}
 size_t coef_size(int precision, int coef16) {
   if (precision == TW_FP64) return 8;
   if (precision == TW_FP32) return 4;
@@ -170,10 +175,11 @@ int check_config(const tw_config* c, std::string* err) {
   for (int a = 0; a < 2; a++)
     if ((c->bc[2 * a] == 'P') != (c->bc[2 * a + 1] == 'P')) return fail(err, TW_EINVAL, "periodic sides must pair");
   if (c->precision < TW_FP64 || c->precision > TW_FP16) return fail(err, TW_EINVAL, "precision");
-  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K) return fail(err, TW_EINVAL, "cpst");
-  if (ref_precision(c->precision) && c->cpst == TW_CPST_K) return fail(err, TW_EINVAL, "cpst K needs 16-bit storage");
-  if (!ref_precision(c->precision) && c->cpst != TW_CPST_0 && c->cpst != TW_CPST_K)
-    return fail(err, TW_EINVAL, "16-bit storage supports cpst 0 or K");
+  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K8) return fail(err, TW_EINVAL, "cpst");
+  const bool k16 = c->cpst == TW_CPST_K || c->cpst == TW_CPST_K8;
+  if (ref_precision(c->precision) && k16) return fail(err, TW_EINVAL, "cpst K and K8 need 16-bit storage");
+  if (!ref_precision(c->precision) && c->cpst != TW_CPST_0 && !k16)
+    return fail(err, TW_EINVAL, "16-bit storage supports cpst 0, K or K8");
   if (c->energy_form == 1 && (c->physics != TW_ACOUSTIC || c->bc[0] != 'P' || c->bc[2] != 'P'))
     return fail(err, TW_EINVAL, "energy_form 1 is the periodic acoustic reference form");
   if (c->energy_form != 0 && c->energy_form != 1) return fail(err, TW_EINVAL, "energy_form");
@@ -560,8 +566,9 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     shape_of(c, field_grid(f), &d.rows, &d.cols);
     d.pitch = pitch_of(d.cols, pad);
     const size_t bytes = (size_t)d.rows * d.pitch * es;
+    const size_t rbytes = (size_t)d.rows * d.pitch * array_size(c, 1);
     if ((e = dalloc(s, &d.s, bytes))) return bail(e);
-    if (rmode(c->cpst) && (e = dalloc(s, &d.r, bytes))) return bail(e);
+    if (rmode(c->cpst) && (e = dalloc(s, &d.r, rbytes))) return bail(e);
     if (cmode(c->cpst) && (e = dalloc(s, &d.c, bytes))) return bail(e);
   }
   p.fused = p.kver == 3 && tw::fused_ok(p);
@@ -571,7 +578,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
       tw::DevField& d = p.f[f];
       const size_t bytes = (size_t)d.rows * d.pitch * es;
       if ((e = dalloc(s, &d.s2, bytes))) return bail(e);
-      if (rmode(c->cpst) && (e = dalloc(s, &d.r2, bytes))) return bail(e);
+      if (rmode(c->cpst) && (e = dalloc(s, &d.r2, (size_t)d.rows * d.pitch * array_size(c, 1)))) return bail(e);
     }
   const size_t ks = coef_size(c->precision, c->coef16);
   for (int k = 0; k < 6; k++) {
@@ -1047,7 +1054,7 @@ static int copy_field(tw_solver* s, int32_t field, int which, double* out, const
   (void)hipSetDevice(s->cfg.device);
   const tw::DevField& d = s->plan.f[field];
   void* dev = which == 0 ? d.s : (which == 1 ? d.r : d.c);
-  const size_t es = state_size(s->cfg.precision);
+  const size_t es = array_size(&s->cfg, which);
   HIPCHK(s, hipStreamSynchronize(s->stream));
   if (!dev) {
     if (out) memset(out, 0, (size_t)d.rows * d.cols * 8);
@@ -1056,8 +1063,13 @@ static int copy_field(tw_solver* s, int32_t field, int which, double* out, const
   std::vector<uint8_t> buf((size_t)d.rows * d.cols * es);
   if (out) {
     HIPCHK(s, hipMemcpy2D(buf.data(), d.cols * es, dev, d.pitch * es, d.cols * es, d.rows, hipMemcpyDeviceToHost));
-    from_storage(s->cfg.precision, buf.data(), out, (size_t)d.rows * d.cols);
+    if (es == 1) {  // K8 residual codes
+      for (size_t i = 0; i < buf.size(); i++) out[i] = (double)(int8_t)buf[i];
+    } else {
+      from_storage(s->cfg.precision, buf.data(), out, (size_t)d.rows * d.cols);
+    }
   } else {
+    if (es == 1) return fail(&s->err, TW_EINVAL, "K8 residual codes are not set from values");
     to_storage(s->cfg.precision, in, buf.data(), (size_t)d.rows * d.cols);
     HIPCHK(s, hipMemcpy2D(dev, d.pitch * es, buf.data(), d.cols * es, d.cols * es, d.rows, hipMemcpyHostToDevice));
     s->ep_valid = false;
@@ -1076,12 +1088,11 @@ int tw_reset(tw_solver* s) {
   if (!s) return fail(nullptr, TW_EINVAL, "null handle");
   (void)hipSetDevice(s->cfg.device);
   HIPCHK(s, hipStreamSynchronize(s->stream));
-  const size_t es = state_size(s->cfg.precision);
   for (int f = 0; f < 6; f++) {
     const tw::DevField& d = s->plan.f[f];
-    const size_t bytes = (size_t)d.rows * d.pitch * es;
-    for (void* p : {d.s, d.r, d.c})
-      if (p) HIPCHK(s, hipMemset(p, 0, bytes));
+    void* arr[3] = {d.s, d.r, d.c};
+    for (int w = 0; w < 3; w++)
+      if (arr[w]) HIPCHK(s, hipMemset(arr[w], 0, (size_t)d.rows * d.pitch * array_size(&s->cfg, w)));
   }
   const size_t nt = (size_t)(s->cfg.nt_max > 0 ? s->cfg.nt_max : 1);
   for (double* p : {s->plan.ek, s->plan.ep0, s->plan.ep1}) HIPCHK(s, hipMemset(p, 0, nt * 8));
@@ -1098,11 +1109,11 @@ static const uint64_t kMagic = 0x3158414d56415754ull;  // "TWAVMAX1"
 int tw_checkpoint_size(tw_solver* s, size_t* bytes) {
   if (!s || !bytes) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
   size_t n = 3 * sizeof(uint64_t);
-  const size_t es = state_size(s->cfg.precision);
   for (int f = 0; f < 6; f++) {
     const tw::DevField& d = s->plan.f[f];
-    for (void* p : {d.s, d.r, d.c})
-      if (p) n += (size_t)d.rows * d.cols * es;
+    void* arr[3] = {d.s, d.r, d.c};
+    for (int w = 0; w < 3; w++)
+      if (arr[w]) n += (size_t)d.rows * d.cols * array_size(&s->cfg, w);
   }
   *bytes = n;
   return TW_OK;
@@ -1120,11 +1131,13 @@ int tw_checkpoint_save(tw_solver* s, void* buf, size_t bytes) {
   h[1] = ((uint64_t)s->cfg.precision << 32) | (uint64_t)s->cfg.cpst;
   h[2] = (uint64_t)s->it;
   uint8_t* o = (uint8_t*)buf + 3 * sizeof(uint64_t);
-  const size_t es = state_size(s->cfg.precision);
   for (int f = 0; f < 6; f++) {
     const tw::DevField& d = s->plan.f[f];
-    for (void* p : {d.s, d.r, d.c}) {
+    void* arr[3] = {d.s, d.r, d.c};
+    for (int w = 0; w < 3; w++) {
+      void* p = arr[w];
       if (!p) continue;
+      const size_t es = array_size(&s->cfg, w);
       HIPCHK(s, hipMemcpy2D(o, d.cols * es, p, d.pitch * es, d.cols * es, d.rows, hipMemcpyDeviceToHost));
       o += (size_t)d.rows * d.cols * es;
     }
@@ -1143,11 +1156,13 @@ int tw_checkpoint_load(tw_solver* s, const void* buf, size_t bytes) {
   (void)hipSetDevice(s->cfg.device);
   HIPCHK(s, hipStreamSynchronize(s->stream));
   const uint8_t* o = (const uint8_t*)buf + 3 * sizeof(uint64_t);
-  const size_t es = state_size(s->cfg.precision);
   for (int f = 0; f < 6; f++) {
     const tw::DevField& d = s->plan.f[f];
-    for (void* p : {d.s, d.r, d.c}) {
+    void* arr[3] = {d.s, d.r, d.c};
+    for (int w = 0; w < 3; w++) {
+      void* p = arr[w];
       if (!p) continue;
+      const size_t es = array_size(&s->cfg, w);
       HIPCHK(s, hipMemcpy2D(p, d.pitch * es, o, d.cols * es, d.cols * es, d.rows, hipMemcpyHostToDevice));
       o += (size_t)d.rows * d.cols * es;
     }
@@ -1613,13 +1628,13 @@ int batch_launch(tw_batch* b, int nsteps, hipEvent_t t0, hipEvent_t t1) {
 int batch_state(tw_batch* b, std::vector<void*>& bufs, bool save) {
   size_t k = 0;
   for (tw_solver* s : b->sv) {
-    const size_t es = state_size(s->cfg.precision);
     for (int f = 0; f < tw::NF; f++) {
       const tw::DevField& d = s->plan.f[f];
       void* arr[3] = {d.s, d.r, d.c};
-      for (void* a : arr) {
+      for (int w = 0; w < 3; w++) {
+        void* a = arr[w];
         if (!a) continue;
-        const size_t bytes = (size_t)d.rows * d.pitch * es;
+        const size_t bytes = (size_t)d.rows * d.pitch * array_size(&s->cfg, w);
         if (save) {
           if (k == bufs.size()) {
             void* q = nullptr;

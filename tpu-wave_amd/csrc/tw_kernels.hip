@@ -73,6 +73,56 @@ __device__ __forceinline__ CT kld(KT v) {
   return (CT)St<KT>::ld(v);
 }
 
+// ------------------------------------------------------------ 8-bit residual
+// TW_CPST_K8 keeps the rounding residual of a 16-bit store in one byte per
+// value: an integer code q in [-127, 127], residual = q * 2^(max(e, 1) - bias)
+// with e the exponent field of the stored value, i.e. q units of ulp / 256
+// (bias 142 for BF16, whose ulp is 2^(e - 134); 33 for FP16, ulp 2^(e - 25)).
+// Both scalings are exact; the code is the residual in those units rounded to
+// nearest even and clamped.  In registers a code travels in the storage
+// type's 16 bits as a sign-extended integer.
+template <class ST>
+__device__ __forceinline__ int k8_shift(ST hi) {
+  const uint32_t b = __builtin_bit_cast(uint16_t, hi);
+  if constexpr (std::is_same_v<ST, bf16>) {
+    const int e = (b >> 7) & 0xff;
+    return (e > 1 ? e : 1) - 142;
+  } else {
+    const int e = (b >> 10) & 0x1f;
+    return (e > 1 ? e : 1) - 33;
+  }
+}
+template <class ST>
+__device__ __forceinline__ float k8_dec(ST q, ST hi) {
+  return __builtin_ldexpf((float)__builtin_bit_cast(int16_t, q), k8_shift(hi));
+}
+template <class ST>
+__device__ __forceinline__ ST k8_enc(float x, ST hi) {
+  const float q = __builtin_rintf(__builtin_ldexpf(x, -k8_shift(hi)));
+  const float c = __builtin_fminf(__builtin_fmaxf(q, -127.f), 127.f);
+  return __builtin_bit_cast(ST, (int16_t)(int)c);
+}
+// residual mode of a compensation variant: 1 the 16-bit residual, 2 the byte code
+__host__ __device__ __forceinline__ int kmode_of(int cpst) {
+  return cpst == TW_CPST_K ? 1 : (cpst == TW_CPST_K8 ? 2 : 0);
+}
+// companion element q of a per-point access; byte codes widen to the register form
+template <class ST>
+__device__ __forceinline__ ST rd_res(const ST* r, int64_t q, int km) {
+  if constexpr (sizeof(ST) == 2)
+    if (km == 2) return __builtin_bit_cast(ST, (int16_t)((const int8_t*)r)[q]);
+  return r[q];
+}
+template <class ST>
+__device__ __forceinline__ void wr_res(ST* r, int64_t q, ST v, int km) {
+  if constexpr (sizeof(ST) == 2)
+    if (km == 2) {
+      ((int8_t*)r)[q] = (int8_t)__builtin_bit_cast(int16_t, v);
+      return;
+    }
+  r[q] = v;
+}
+
 // ------------------------------------------------------------ typed plan
 template <class CT>
 struct TabT {
@@ -321,6 +371,8 @@ template <class ST, class CT>
 __device__ __forceinline__ CT acc_init(const F<ST>& X, int64_t q, int cpst) {
   if (cpst == TW_CPST_3R || cpst == TW_CPST_M3R || cpst == TW_CPST_6R || cpst == TW_CPST_K)
     return (CT)St<ST>::ld(X.r[q]);
+  if constexpr (sizeof(ST) == 2)
+    if (cpst == TW_CPST_K8) return (CT)k8_dec(rd_res(X.r, q, 2), X.s[q]);
   return (CT)0;
 }
 
@@ -390,6 +442,7 @@ __device__ __forceinline__ void update(const F<ST>& X, int64_t q, CT acc, int cp
     const ST hi = S::st(t);
     *snew = hi;
     if (cpst == TW_CPST_K) *rnew = S::st(t - S::ld(hi));
+    if (cpst == TW_CPST_K8) *rnew = k8_enc(t - S::ld(hi), hi);
   }
 }
 
@@ -400,7 +453,13 @@ __device__ __forceinline__ void add_source(ST* snew, ST* rnew, double inc, int k
   if constexpr (sizeof(ST) >= 4) {
     *snew = (ST)((double)*snew + inc);
   } else {
-    if (kmode) {
+    if (kmode == 2) {
+      const double v = (double)S::ld(*snew) + (double)k8_dec(*rnew, *snew) + inc;
+      const float vf = (float)v;
+      const ST hi = S::st(vf);
+      *snew = hi;
+      *rnew = k8_enc(vf - S::ld(hi), hi);
+    } else if (kmode) {
       const double v = (double)S::ld(*snew) + (double)S::ld(*rnew) + inc;
       const float vf = (float)v;
       const ST hi = S::st(vf);
@@ -416,7 +475,8 @@ template <class ST, class CT>
 __device__ __forceinline__ CT value_of(ST s, ST r, int kmode) {
   CT v = (CT)St<ST>::ld(s);
   if constexpr (sizeof(ST) < 4) {
-    if (kmode) v = v + (CT)St<ST>::ld(r);
+    if (kmode == 2) v = v + (CT)k8_dec(r, s);
+    else if (kmode) v = v + (CT)St<ST>::ld(r);
   }
   return v;
 }
@@ -488,13 +548,13 @@ __device__ void do_pending(const Stat<ST, KT, CT>& A, const Dyn& D, double* sh) 
       if (P.out1) P.out1[P.it1] = v;
     }
   }
-  const int kmode = A.cpst == TW_CPST_K;
+  const int kmode = kmode_of(A.cpst);
   for (int r = tid; r < P.nrcv; r += nth) {
     const Receiver rc = P.rcv[r];
     const F<ST>& X = A.f[rc.field];
     const int64_t q = X.at(rc.ix, rc.iy);
     ST rr = X.s[q];
-    if (kmode) rr = X.r[q];
+    if (kmode) rr = rd_res(X.r, q, kmode);
     D.rec[(int64_t)rc.row * D.nt_max + P.rcv_it] = (double)value_of<ST, CT>(X.s[q], rr, kmode);
   }
 }
@@ -527,7 +587,7 @@ __device__ __forceinline__ bool map_point(const Rects& R, int bx, int by, int* i
 template <class ST, class KT, class CT>
 __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn& D, int ix, int iy, bool inr) {
   const int el = A.physics == TW_ELASTIC;
-  const int kmode = A.cpst == TW_CPST_K;
+  const int kmode = kmode_of(A.cpst);
   double e = 0.0;
   // ---- vx on NM: d/dx of sxx (elastic) or p, + d/dy of sxy (elastic)
   {
@@ -547,12 +607,12 @@ __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         if (!A.pery) acc = sat_add(A, acc, TW_VX, TW_SXY, 1, 0, ix, iy, own, b);
       }
       ST sn = X.s[q], rn = sn;
-      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) rn = X.r[q];
+      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) rn = rd_res(X.r, q, kmode);
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
       if (D.src_on && D.src_grid == TW_GRID_NM && ix == D.src_ix && iy == D.src_iy)
         add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
       X.so[q] = sn;
-      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.ro[q] = rn;
+      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) wr_res(X.ro, q, rn, kmode);
       if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 1, 0, ix, iy, X.rows, X.cols);
     }
   }
@@ -574,12 +634,12 @@ __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
       acc = acc + d * b;
       if (!A.pery) acc = sat_add(A, acc, TW_VY, pyf, 1, 1, ix, iy, own, b);
       ST sn = X.s[q], rn = sn;
-      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) rn = X.r[q];
+      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) rn = rd_res(X.r, q, kmode);
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
       if (D.src_on && D.src_grid == TW_GRID_MN && ix == D.src_ix && iy == D.src_iy)
         add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
       X.so[q] = sn;
-      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) X.ro[q] = rn;
+      if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) wr_res(X.ro, q, rn, kmode);
       if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), b, 0, 1, ix, iy, X.rows, X.cols);
     }
   }
@@ -609,7 +669,7 @@ __global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
 // one point of the stress half step (the body of the per-point kernels); returns its energy term
 template <class ST, class KT, class CT>
 __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn& D, int ix, int iy, bool inr) {
-  const int kmode = A.cpst == TW_CPST_K;
+  const int kmode = kmode_of(A.cpst);
   const int rmode = kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R;
   double e = 0.0;
   if (A.physics == TW_ELASTIC) {
@@ -624,12 +684,12 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         d = ddy<ST, CT>(A.f[TW_VX], 1, ix, iy, X.cols, A.pery, A.my, A.t);
         acc = acc + d * mu;
         ST sn = X.s[q], rn = sn;
-        if (rmode) rn = X.r[q];
+        if (rmode) rn = rd_res(X.r, q, kmode);
         update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
         if (D.src_on && D.src_grid == TW_GRID_NN && ix == D.src_ix && iy == D.src_iy)
           add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
         X.so[q] = sn;
-        if (rmode) X.ro[q] = rn;
+        if (rmode) wr_res(X.ro, q, rn, kmode);
         if (D.energy) e += energy_pt(A, value_of<ST, CT>(sn, rn, kmode), mu, 1, 1, ix, iy, X.rows, X.cols);
       }
     }
@@ -652,8 +712,8 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         a1 = a1 + w * tmu;
         ST s0 = X0.s[q], r0 = s0, s1 = X1.s[q], r1 = s1;
         if (rmode) {
-          r0 = X0.r[q];
-          r1 = X1.r[q];
+          r0 = rd_res(X0.r, q, kmode);
+          r1 = rd_res(X1.r, q, kmode);
         }
         update<ST, CT>(X0, q, a0, A.cpst, &s0, &r0);
         update<ST, CT>(X1, q, a1, A.cpst, &s1, &r1);
@@ -664,8 +724,8 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         X0.so[q] = s0;
         X1.so[q] = s1;
         if (rmode) {
-          X0.ro[q] = r0;
-          X1.ro[q] = r1;
+          wr_res(X0.ro, q, r0, kmode);
+          wr_res(X1.ro, q, r1, kmode);
         }
         if (D.energy) {
           const double L = (double)lam, M2 = (double)tmu;
@@ -691,12 +751,12 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
       d = ddy<ST, CT>(A.f[TW_VY], 0, ix, iy, X.cols, A.pery, A.my, A.t);
       acc = acc + d * kap;
       ST sn = X.s[q], rn = sn;
-      if (rmode) rn = X.r[q];
+      if (rmode) rn = rd_res(X.r, q, kmode);
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
       if (D.src_on && D.src_grid == TW_GRID_MM && ix == D.src_ix && iy == D.src_iy)
         add_source<ST, CT>(&sn, &rn, D.src_inc, kmode);
       X.so[q] = sn;
-      if (rmode) X.ro[q] = rn;
+      if (rmode) wr_res(X.ro, q, rn, kmode);
       if (D.energy) {
         const CT v = value_of<ST, CT>(sn, rn, kmode);
         if (A.eform == 1) e += energy_pt(A, v, kap, 0, 0, ix, iy, X.rows, X.cols);
@@ -762,7 +822,7 @@ __global__ void __launch_bounds__(256) k_stress_energy(Args<ST, KT, CT> A) {
   __shared__ double sh[8];
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int ix = blockIdx.y * blockDim.y + threadIdx.y;
-  const int kmode = A.cpst == TW_CPST_K;
+  const int kmode = kmode_of(A.cpst);
   double e = 0.0;
   if (A.physics == TW_ELASTIC) {
     const F<ST>& X = A.f[TW_SXY];
@@ -770,7 +830,7 @@ __global__ void __launch_bounds__(256) k_stress_energy(Args<ST, KT, CT> A) {
       const int64_t q = X.at(ix, iy);
       const CT mu = kld<KT, CT>(A.k[TW_C_MU].get(ix, iy));
       ST r = X.s[q];
-      if (kmode) r = X.r[q];
+      if (kmode) r = rd_res(X.r, q, kmode);
       e += energy_pt(A, value_of<ST, CT>(X.s[q], r, kmode), mu, 1, 1, ix, iy, X.rows, X.cols);
     }
     const F<ST>& X0 = A.f[TW_SXX];
@@ -786,8 +846,8 @@ __global__ void __launch_bounds__(256) k_stress_energy(Args<ST, KT, CT> A) {
       p2 = (p2 * hx) * hy;
       ST r0 = X0.s[q], r1 = X1.s[q];
       if (kmode) {
-        r0 = X0.r[q];
-        r1 = X1.r[q];
+        r0 = rd_res(X0.r, q, kmode);
+        r1 = rd_res(X1.r, q, kmode);
       }
       const double sxx = (double)value_of<ST, CT>(X0.s[q], r0, kmode);
       const double syy = (double)value_of<ST, CT>(X1.s[q], r1, kmode);
@@ -799,7 +859,7 @@ __global__ void __launch_bounds__(256) k_stress_energy(Args<ST, KT, CT> A) {
       const int64_t q = X.at(ix, iy);
       const CT kap = kld<KT, CT>(A.k[TW_C_KAPPA].get(ix, iy));
       ST r = X.s[q];
-      if (kmode) r = X.r[q];
+      if (kmode) r = rd_res(X.r, q, kmode);
       e += energy_pt(A, value_of<ST, CT>(X.s[q], r, kmode), kap, 0, 0, ix, iy, X.rows, X.cols);
     }
   }
@@ -1036,6 +1096,8 @@ struct FR {
   __device__ __forceinline__ gptr<ST> rrow(int i) const { return r + (int64_t)i * pitch; }
   __device__ __forceinline__ gptr<ST> orow(int i) const { return so + (int64_t)i * pitch; }
   __device__ __forceinline__ gptr<ST> orrow(int i) const { return ro + (int64_t)i * pitch; }
+  // residual rows of the K8 variant: one byte per value, same pitch in elements
+  __device__ __forceinline__ gptr<int8_t> brow(int i) const { return (gptr<int8_t>)r + (int64_t)i * pitch; }
 };
 
 template <class ST, class CT>
@@ -1090,20 +1152,80 @@ __device__ __forceinline__ double energy_core(int eform, double dx, CT v, CT prm
   return s * w * s;
 }
 
-template <class ST>
-__device__ __forceinline__ void store4(const FR<ST>& X, int r, int c, const V4<ST>& sn, const V4<ST>& rn, int rm, int full,
+// compensation variant fixed at compile time: rm, a residual array is read
+// and written; km, the residual mode of value_of / add_source (kmode_of)
+template <int CP>
+struct CPT {
+  static constexpr bool rm =
+      CP == TW_CPST_3R || CP == TW_CPST_M3R || CP == TW_CPST_6R || CP == TW_CPST_K || CP == TW_CPST_K8;
+  static constexpr int km = CP == TW_CPST_K ? 1 : (CP == TW_CPST_K8 ? 2 : 0);
+};
+
+// a lane's four residual values of row `row` at column c (multiple of 4):
+// storage-type elements, or four byte codes for K8
+template <class ST, int CP>
+__device__ __forceinline__ V4<ST> ld_r4(const FR<ST>& X, int row, int c) {
+  if constexpr (CP == TW_CPST_K8) {
+    typedef __attribute__((address_space(1))) const uint32_t* Q;
+    const uint32_t w = *(Q)(X.brow(row) + c);
+    V4<ST> o;
+#pragma unroll
+    for (int j = 0; j < 4; j++) o.v[j] = __builtin_bit_cast(ST, (int16_t)(int8_t)(w >> (8 * j)));
+    return o;
+  } else {
+    return ld_v4<ST>(X.rrow(row) + c);
+  }
+}
+
+// store of a lane's four residual values, element-wise where only some columns are in
+template <class ST, int CP>
+__device__ __forceinline__ void st_r4(const FR<ST>& X, int row, int c, const V4<ST>& v, bool full4, const bool* in) {
+  if constexpr (CP == TW_CPST_K8) {
+    const auto p = X.brow(row) + c;
+    if (full4) {
+      uint32_t w = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) w |= ((uint32_t)__builtin_bit_cast(uint16_t, v.v[j]) & 0xffu) << (8 * j);
+      typedef __attribute__((address_space(1))) uint32_t* Q;
+      *(Q)p = w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (in[j]) p[j] = (int8_t)__builtin_bit_cast(int16_t, v.v[j]);
+    }
+  } else {
+    if (full4) {
+      st_v4<ST>(X.rrow(row) + c, v);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (in[j]) st_1<ST>(X.rrow(row) + c + j, v.v[j]);
+    }
+  }
+}
+
+// accumulator start of a compensated update: R or the decoded residual, else zero
+template <class ST, class CT, int CP>
+__device__ __forceinline__ CT acc0(ST r, ST s) {
+  if constexpr (CP == TW_CPST_K8)
+    return (CT)k8_dec(r, s);
+  else if constexpr (CPT<CP>::rm)
+    return (CT)St<ST>::ld(r);
+  else
+    return (CT)0;
+}
+
+template <class ST, int CP>
+__device__ __forceinline__ void store4(const FR<ST>& X, int r, int c, const V4<ST>& sn, const V4<ST>& rn, int full,
                                        const bool in[4]) {
   if (full) {
     st_v4<ST>(X.row(r) + c, sn);
-    if (rm) st_v4<ST>(X.rrow(r) + c, rn);
   } else {
 #pragma unroll
     for (int j = 0; j < 4; j++)
-      if (in[j]) {
-        st_1<ST>(X.row(r) + c + j, sn.v[j]);
-        if (rm) st_1<ST>(X.rrow(r) + c + j, rn.v[j]);
-      }
+      if (in[j]) st_1<ST>(X.row(r) + c + j, sn.v[j]);
   }
+  if constexpr (CPT<CP>::rm) st_r4<ST, CP>(X, r, c, rn, full, in);
 }
 
 // Leapfrog update in registers with the compensation variant fixed at
@@ -1142,14 +1264,9 @@ __device__ __forceinline__ void upd(ST& s, ST& r, CT acc) {
     const ST hi = S::st(t);
     s = hi;
     if constexpr (CP == TW_CPST_K) r = S::st(t - S::ld(hi));
+    if constexpr (CP == TW_CPST_K8) r = k8_enc(t - S::ld(hi), hi);
   }
 }
-
-template <int CP>
-struct CPT {
-  static constexpr bool rm = CP == TW_CPST_3R || CP == TW_CPST_M3R || CP == TW_CPST_6R || CP == TW_CPST_K;
-  static constexpr bool k = CP == TW_CPST_K;
-};
 
 // 12 consecutive segment values around the lane's 4 columns: out[i] = column c - 4 + i
 template <class CT>
@@ -1192,7 +1309,7 @@ __device__ __forceinline__ void src4(V4<ST>& sn, V4<ST>& rn, const Dyn& D, int g
 #pragma unroll
   for (int q = 0; q < 4; q++) {  // selects only: no dynamic index into the register arrays
     ST s1 = sn.v[q], r1 = rn.v[q];
-    add_source<ST, CT>(&s1, &r1, D.src_inc, CPT<CP>::k);
+    add_source<ST, CT>(&s1, &r1, D.src_inc, CPT<CP>::km);
     const bool hit = q == j;
     sn.v[q] = hit ? s1 : sn.v[q];
     rn.v[q] = hit ? r1 : rn.v[q];
@@ -1254,14 +1371,14 @@ __device__ __forceinline__ void seg_f(CT* seg, const FR<ST>& P, int row, int bas
     seg_fill_r<ST, CT>(seg, P, row, base, lane, periodic, M);
 }
 
-template <bool FULL, class ST>
-__device__ __forceinline__ void store_f(const FR<ST>& X, int r, int c, const V4<ST>& sn, const V4<ST>& rn, bool rm,
-                                        int full, const bool in[4]) {
+template <bool FULL, class ST, int CP>
+__device__ __forceinline__ void store_f(const FR<ST>& X, int r, int c, const V4<ST>& sn, const V4<ST>& rn, int full,
+                                        const bool in[4]) {
   if constexpr (FULL) {
     st_v4<ST>(X.row(r) + c, sn);
-    if (rm) st_v4<ST>(X.rrow(r) + c, rn);
+    if constexpr (CPT<CP>::rm) st_r4<ST, CP>(X, r, c, rn, true, in);
   } else {
-    store4<ST>(X, r, c, sn, rn, rm, full, in);
+    store4<ST, CP>(X, r, c, sn, rn, full, in);
   }
 }
 
@@ -1383,7 +1500,8 @@ __device__ __forceinline__ double e_mm(CT lam, CT tmu, CT sxx, CT syy) {
 template <class ST, class KT, class CT, int PH, int CP>
 __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
                                           int by, int nbx) {
-  constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
+  constexpr bool RM = CPT<CP>::rm;
+  constexpr int KM = CPT<CP>::km;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // sxy row (y partner of vx, elastic)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // syy or p row (y partner of vy)
   __shared__ double sh[8];
@@ -1456,12 +1574,12 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         ld4<KT, CT>(bxp + (int64_t)r * bxs, c, b);
         V4<ST> sn = ld_v4<ST>(VX.row(r) + c);
         V4<ST> rn = sn;
-        if constexpr (RM) rn = ld_v4<ST>(VX.rrow(r) + c);
+        if constexpr (RM) rn = ld_r4<ST, CP>(VX, r, c);
         CT ya[12];
         if constexpr (PH == TW_ELASTIC) seg12<CT>(segA[buf], lane, ya);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+          CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
           acc = acc + dx_win<CT>(wa, j, st) * b[j];
           if constexpr (PH == TW_ELASTIC) acc = acc + dy12<CT>(ya, j, 0, st) * b[j];
           upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
@@ -1475,19 +1593,19 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
             e += inx[j] ? t : 0.0;
           }
         }
-        store_f<FULL, ST>(VX, r, c, sn, rn, RM, fullx, inx);
+        store_f<FULL, ST, CP>(VX, r, c, sn, rn, fullx, inx);
       }
       if (anyy && vy_row) {  // ---- vy (MN)
         CT b[4];
         ld4<KT, CT>(byp + (int64_t)r * bys, c, b);
         V4<ST> sn = ld_v4<ST>(VY.row(r) + c);
         V4<ST> rn = sn;
-        if constexpr (RM) rn = ld_v4<ST>(VY.rrow(r) + c);
+        if constexpr (RM) rn = ld_r4<ST, CP>(VY, r, c);
         CT yb[12];
         seg12<CT>(segB[buf], lane, yb);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+          CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
           if constexpr (PH == TW_ELASTIC) acc = acc + dx_win<CT>(wb, j, st) * b[j];
           acc = acc + dy12<CT>(yb, j, 1, st) * b[j];
           upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
@@ -1501,7 +1619,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
             e += iny[j] ? t : 0.0;
           }
         }
-        store_f<FULL, ST>(VY, r, c, sn, rn, RM, fully, iny);
+        store_f<FULL, ST, CP>(VY, r, c, sn, rn, fully, iny);
       }
       buf ^= 1;
     }
@@ -1545,11 +1663,11 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       const V4<KT> kx = ld_v4<KT>(bxp + (int64_t)rx * bxs + cl);
       V4<ST> sx = ld_v4<ST>(VX.row(rx) + cl);
       V4<ST> qx = sx;
-      if constexpr (RM) qx = ld_v4<ST>(VX.rrow(rx) + cl);
+      if constexpr (RM) qx = ld_r4<ST, CP>(VX, rx, cl);
       const V4<KT> ky = ld_v4<KT>(byp + (int64_t)ry * bys + cl);
       V4<ST> sy = ld_v4<ST>(VY.row(ry) + cl);
       V4<ST> qy = sy;
-      if constexpr (RM) qy = ld_v4<ST>(VY.rrow(ry) + cl);
+      if constexpr (RM) qy = ld_r4<ST, CP>(VY, ry, cl);
       // ---- segments
       cvt_v4<ST, CT>(nA, a3);
       if constexpr (PH == TW_ELASTIC) {
@@ -1577,7 +1695,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         if constexpr (PH == TW_ELASTIC) seg12<CT>(segA[buf], lane, ya);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          CT acc = RM ? (CT)St<ST>::ld(qx.v[j]) : (CT)0;
+          CT acc = acc0<ST, CT, CP>(qx.v[j], sx.v[j]);
           acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * b[j];
           if constexpr (PH == TW_ELASTIC) acc = acc + dy12<CT>(ya, j, 0, st) * b[j];
           upd<ST, CT, CP>(sx.v[j], qx.v[j], acc);
@@ -1588,7 +1706,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
           for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sx.v[j], qx.v[j], KM), b[j]) : 0.0;
         }
         st_m<ST>(VX.row(r), c, sx, full4, in);
-        if constexpr (RM) st_m<ST>(VX.rrow(r), c, qx, full4, in);
+        if constexpr (RM) st_r4<ST, CP>(VX, r, c, qx, full4, in);
       }
       if (vy_row) {  // ---- vy (MN)
         CT b[4];
@@ -1597,7 +1715,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         seg12<CT>(segB[buf], lane, yb);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          CT acc = RM ? (CT)St<ST>::ld(qy.v[j]) : (CT)0;
+          CT acc = acc0<ST, CT, CP>(qy.v[j], sy.v[j]);
           if constexpr (PH == TW_ELASTIC) acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * b[j];
           acc = acc + dy12<CT>(yb, j, 1, st) * b[j];
           upd<ST, CT, CP>(sy.v[j], qy.v[j], acc);
@@ -1608,7 +1726,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
           for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sy.v[j], qy.v[j], KM), b[j]) : 0.0;
         }
         st_m<ST>(VY.row(r), c, sy, full4, in);
-        if constexpr (RM) st_m<ST>(VY.rrow(r), c, qy, full4, in);
+        if constexpr (RM) st_r4<ST, CP>(VY, r, c, qy, full4, in);
       }
     };
     for (int r = x0;;) {
@@ -1650,7 +1768,8 @@ k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C, int nbx, int ncor
 template <class ST, class KT, class CT, int PH, int CP>
 __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
                                           int by, int nbx) {
-  constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
+  constexpr bool RM = CPT<CP>::rm;
+  constexpr int KM = CPT<CP>::km;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // vx row (y partner of sxy)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // vy row (y partner of MM / p)
   __shared__ double sh[8];
@@ -1728,12 +1847,12 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           ld4<KT, CT>(kmp + (int64_t)r * kms, c, mu);
           V4<ST> sn = ld_v4<ST>(XN.row(r) + c);
           V4<ST> rn = sn;
-          if constexpr (RM) rn = ld_v4<ST>(XN.rrow(r) + c);
+          if constexpr (RM) rn = ld_r4<ST, CP>(XN, r, c);
           CT ya[12];
           seg12<CT>(segA[buf], lane, ya);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+            CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
             acc = acc + dx_win<CT>(wa, j, st) * mu[j];
             acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
             upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
@@ -1747,7 +1866,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
               e += inn[j] ? t : 0.0;
             }
           }
-          store_f<FULL, ST>(XN, r, c, sn, rn, RM, fulln, inn);
+          store_f<FULL, ST, CP>(XN, r, c, sn, rn, fulln, inn);
         }
       }
       if (anym && mm_row) {
@@ -1761,13 +1880,13 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           V4<ST> n1 = ld_v4<ST>(X1.row(r) + c);
           V4<ST> m0 = n0, m1 = n1;
           if constexpr (RM) {
-            m0 = ld_v4<ST>(X0.rrow(r) + c);
-            m1 = ld_v4<ST>(X1.rrow(r) + c);
+            m0 = ld_r4<ST, CP>(X0, r, c);
+            m1 = ld_r4<ST, CP>(X1, r, c);
           }
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            CT a0 = RM ? (CT)St<ST>::ld(m0.v[j]) : (CT)0;
-            CT a1 = RM ? (CT)St<ST>::ld(m1.v[j]) : (CT)0;
+            CT a0 = acc0<ST, CT, CP>(m0.v[j], n0.v[j]);
+            CT a1 = acc0<ST, CT, CP>(m1.v[j], n1.v[j]);
             const CT v = dx_win<CT>(wb, j, st);
             a0 = a0 + v * lam[j];
             a1 = a1 + v * lam[j];
@@ -1789,17 +1908,17 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
               e += inm[j] ? t : 0.0;
             }
           }
-          store_f<FULL, ST>(X0, r, c, n0, m0, RM, fullm, inm);
-          store_f<FULL, ST>(X1, r, c, n1, m1, RM, fullm, inm);
+          store_f<FULL, ST, CP>(X0, r, c, n0, m0, fullm, inm);
+          store_f<FULL, ST, CP>(X1, r, c, n1, m1, fullm, inm);
         } else {  // ---- p (MM, acoustic)
           CT kap[4];
           ld4<KT, CT>(k0p + (int64_t)r * k0s, c, kap);
           V4<ST> sn = ld_v4<ST>(X0.row(r) + c);
           V4<ST> rn = sn;
-          if constexpr (RM) rn = ld_v4<ST>(X0.rrow(r) + c);
+          if constexpr (RM) rn = ld_r4<ST, CP>(X0, r, c);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            CT acc = RM ? (CT)St<ST>::ld(rn.v[j]) : (CT)0;
+            CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
             acc = acc + dx_win<CT>(wb, j, st) * kap[j];
             acc = acc + dy12<CT>(yb, j, 0, st) * kap[j];
             upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
@@ -1813,7 +1932,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
               e += inm[j] ? t : 0.0;
             }
           }
-          store_f<FULL, ST>(X0, r, c, sn, rn, RM, fullm, inm);
+          store_f<FULL, ST, CP>(X0, r, c, sn, rn, fullm, inm);
         }
       }
       buf ^= 1;
@@ -1853,7 +1972,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         kn = ld_v4<KT>(kmp + (int64_t)r * kms + cl);
         sn = ld_v4<ST>(XN.row(r) + cl);
         qn = sn;
-        if constexpr (RM) qn = ld_v4<ST>(XN.rrow(r) + cl);
+        if constexpr (RM) qn = ld_r4<ST, CP>(XN, r, cl);
       } else {
         oB = ld_v4<ST>(VY.row(rm) + cl);  // vy row r (no window in the acoustic case)
       }
@@ -1864,11 +1983,11 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       if constexpr (PH == TW_ELASTIC) k1 = ld_v4<KT>(k1p + (int64_t)rm * k1s + cl);
       V4<ST> s0 = ld_v4<ST>(X0.row(rm) + cl), s1;
       V4<ST> q0 = s0, q1;
-      if constexpr (RM) q0 = ld_v4<ST>(X0.rrow(rm) + cl);
+      if constexpr (RM) q0 = ld_r4<ST, CP>(X0, rm, cl);
       if constexpr (PH == TW_ELASTIC) {
         s1 = ld_v4<ST>(X1.row(rm) + cl);
         q1 = s1;
-        if constexpr (RM) q1 = ld_v4<ST>(X1.rrow(rm) + cl);
+        if constexpr (RM) q1 = ld_r4<ST, CP>(X1, rm, cl);
       }
       // ---- segments
       cvt_v4<ST, CT>(nB, b3);
@@ -1897,7 +2016,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         seg12<CT>(segA[buf], lane, ya);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          CT acc = RM ? (CT)St<ST>::ld(qn.v[j]) : (CT)0;
+          CT acc = acc0<ST, CT, CP>(qn.v[j], sn.v[j]);
           acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * mu[j];
           acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
           upd<ST, CT, CP>(sn.v[j], qn.v[j], acc);
@@ -1908,7 +2027,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sn.v[j], qn.v[j], KM), mu[j]) : 0.0;
         }
         st_m<ST>(XN.row(r), c, sn, full4, in);
-        if constexpr (RM) st_m<ST>(XN.rrow(r), c, qn, full4, in);
+        if constexpr (RM) st_r4<ST, CP>(XN, r, c, qn, full4, in);
       }
       if (mm_row) {
         CT yb[12];
@@ -1919,8 +2038,8 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           cvt_v4<KT, CT>(k1, tmu);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            CT a0 = RM ? (CT)St<ST>::ld(q0.v[j]) : (CT)0;
-            CT a1 = RM ? (CT)St<ST>::ld(q1.v[j]) : (CT)0;
+            CT a0 = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
+            CT a1 = acc0<ST, CT, CP>(q1.v[j], s1.v[j]);
             const CT v = dx4<CT>(b0, b1, b2, b3, j, st);
             a0 = a0 + v * lam[j];
             a1 = a1 + v * lam[j];
@@ -1944,15 +2063,15 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           st_m<ST>(X0.row(r), c, s0, full4, in);
           st_m<ST>(X1.row(r), c, s1, full4, in);
           if constexpr (RM) {
-            st_m<ST>(X0.rrow(r), c, q0, full4, in);
-            st_m<ST>(X1.rrow(r), c, q1, full4, in);
+            st_r4<ST, CP>(X0, r, c, q0, full4, in);
+            st_r4<ST, CP>(X1, r, c, q1, full4, in);
           }
         } else {  // ---- p (MM, acoustic)
           CT kap[4];
           cvt_v4<KT, CT>(k0, kap);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            CT acc = RM ? (CT)St<ST>::ld(q0.v[j]) : (CT)0;
+            CT acc = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
             acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * kap[j];
             acc = acc + dy12<CT>(yb, j, 0, st) * kap[j];
             upd<ST, CT, CP>(s0.v[j], q0.v[j], acc);
@@ -1963,7 +2082,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
             for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(s0.v[j], q0.v[j], KM), kap[j]) : 0.0;
           }
           st_m<ST>(X0.row(r), c, s0, full4, in);
-          if constexpr (RM) st_m<ST>(X0.rrow(r), c, q0, full4, in);
+          if constexpr (RM) st_r4<ST, CP>(X0, r, c, q0, full4, in);
         }
       }
     };
@@ -2095,6 +2214,8 @@ static hipError_t launch_core_p(const Plan& p, int half, const Stat<ST, KT, CT>*
   } else {
     if (p.cpst == TW_CPST_K)
       launch_core_t<ST, KT, CT, PH, TW_CPST_K>(half, sp, d, C, g, b, s, R, nband);
+    else if (p.cpst == TW_CPST_K8)
+      launch_core_t<ST, KT, CT, PH, TW_CPST_K8>(half, sp, d, C, g, b, s, R, nband);
     else
       launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s, R, nband);
   }
@@ -2282,7 +2403,8 @@ constexpr int FSTRIDE = STRIP - 2 * VW;
 template <class ST, class KT, class CT, int CP>
 __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
                                            int by, int nbx) {
-  constexpr bool RM = CPT<CP>::rm, KM = CPT<CP>::k;
+  constexpr bool RM = CPT<CP>::rm;
+  constexpr int KM = CPT<CP>::km;
   using S = St<ST>;
   __shared__ __attribute__((aligned(16))) CT segA[SEG];  // old sxy row v (y partner of vx)
   __shared__ __attribute__((aligned(16))) CT segB[SEG];  // old syy row v (y partner of vy)
@@ -2364,10 +2486,10 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
     const V4<ST> hB = ld_v4<ST>(SYY.row(v) + hc);
     const V4<KT> kx = ld_v4<KT>(bxp + (int64_t)v * bxs + cl);
     V4<ST> sx = ld_v4<ST>(VX.row(v) + cl), qx = sx;
-    if constexpr (RM) qx = ld_v4<ST>(VX.rrow(v) + cl);
+    if constexpr (RM) qx = ld_r4<ST, CP>(VX, v, cl);
     const V4<KT> ky = ld_v4<KT>(byp + (int64_t)v * bys + cl);
     V4<ST> sy = ld_v4<ST>(VY.row(v) + cl), qy = sy;
-    if constexpr (RM) qy = ld_v4<ST>(VY.rrow(v) + cl);
+    if constexpr (RM) qy = ld_r4<ST, CP>(VY, v, cl);
     // stress-row loads: issued with the velocity loads when the variant keeps
     // no residual (its registers allow it); otherwise after the velocity row
     V4<KT> km, k0, k1;
@@ -2377,9 +2499,9 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
       k0 = ld_v4<KT>(lap + (int64_t)sr * las + cl);
       k1 = ld_v4<KT>(tmp_ + (int64_t)sr * tms + cl);
       if constexpr (RM) {
-        qn = ld_v4<ST>(SXY.rrow(sr) + cl);
-        q0 = ld_v4<ST>(SXX.rrow(sr) + cl);
-        q1 = ld_v4<ST>(SYY.rrow(sr) + cl);
+        qn = ld_r4<ST, CP>(SXY, sr, cl);
+        q0 = ld_r4<ST, CP>(SXX, sr, cl);
+        q1 = ld_r4<ST, CP>(SYY, sr, cl);
       }
     };
     if constexpr (!RM) ld_stress();
@@ -2417,7 +2539,7 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
       cvt_v4<ST, CT>(aF, w3);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        CT acc = RM ? (CT)S::ld(qx.v[j]) : (CT)0;
+        CT acc = acc0<ST, CT, CP>(qx.v[j], sx.v[j]);
         acc = acc + dx4<CT>(w0, w1, w2, w3, j, st) * bxv[j];
         acc = acc + dy12<CT>(ya, j, 0, st) * bxv[j];
         upd<ST, CT, CP>(sx.v[j], qx.v[j], acc);
@@ -2428,7 +2550,7 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
       cvt_v4<ST, CT>(nB, w3);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        CT acy = RM ? (CT)S::ld(qy.v[j]) : (CT)0;
+        CT acy = acc0<ST, CT, CP>(qy.v[j], sy.v[j]);
         acy = acy + dx4<CT>(w0, w1, w2, w3, j, st) * byv[j];
         acy = acy + dy12<CT>(yb, j, 1, st) * byv[j];
         upd<ST, CT, CP>(sy.v[j], qy.v[j], acy);
@@ -2484,7 +2606,7 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
         V4<ST> sn = ringB[slot5(s)][lane];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          CT acc = RM ? (CT)S::ld(qn.v[j]) : (CT)0;
+          CT acc = acc0<ST, CT, CP>(qn.v[j], sn.v[j]);
           acc = acc + dx4<CT>(w0, w1, w2, w3, j, st) * mu[j];
           acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
           upd<ST, CT, CP>(sn.v[j], qn.v[j], acc);
@@ -2509,8 +2631,8 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
         V4<ST> s0 = a0, s1 = sqc;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          CT e0 = RM ? (CT)S::ld(q0.v[j]) : (CT)0;
-          CT e1 = RM ? (CT)S::ld(q1.v[j]) : (CT)0;
+          CT e0 = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
+          CT e1 = acc0<ST, CT, CP>(q1.v[j], s1.v[j]);
           const CT vv = dx4<CT>(w0, w1, w2, w3, j, st);
           e0 = e0 + vv * lam[j];
           e1 = e1 + vv * lam[j];
@@ -2572,7 +2694,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(
 // ---- fused step launches
 bool fused_ok(const Plan& p) {
   return p.physics == TW_ELASTIC && !p.perx && !p.pery && p.energy_form == 0 && p.mx >= 12 && p.my >= 12 &&
-         !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C);
+         !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C || p.cpst == TW_CPST_K8);
 }
 
 static void fused_dims(const Plan& p, const Core& C, dim3* g) {
@@ -2739,6 +2861,8 @@ static void launch_batch_core(const Plan& p, int half, const BatchCore* items, i
   } else {
     if (p.cpst == TW_CPST_K)
       go(std::integral_constant<int, TW_CPST_K>{});
+    else if (p.cpst == TW_CPST_K8)
+      go(std::integral_constant<int, TW_CPST_K8>{});
     else
       go(std::integral_constant<int, TW_CPST_0>{});
   }
