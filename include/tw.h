@@ -198,7 +198,8 @@ int tw_read_energy(tw_solver* s, double* ek, double* ep0, double* ep1, int32_t* 
  * cuda/src/cuda/grid.cuh:414-512).  Values are in stored units (not rescaled). */
 int tw_get_field(tw_solver* s, int32_t field, double* out);
 int tw_set_field(tw_solver* s, int32_t field, const double* in);
-/* Companion array (R or C compensation, or the Kahan residual) of a field. */
+/* Companion array (R or C compensation, or the Kahan residual) of a field;
+ * for TW_CPST_K8 the residual codes as integers (-127..127). */
 int tw_get_companion(tw_solver* s, int32_t field, double* out);
 
 /* Zero all fields, companions, records and the step counter. */
@@ -313,7 +314,7 @@ int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* en
  * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:755-772):
  * <root>/<elastic|acoustic>/<medium>/device/grid_size_<num>_<den>/Nt_<Nt>/
  * <fp64|fp32|bf16|fp16>/dt_<dt as %16.15e with the '.' as 'p'>/<cpst>, cpst as
- * 0_0, 3_R, -3_R, 6_R, 3_C, -3_C, 6_C or K.  Writes at most cap bytes. */
+ * 0_0, 3_R, -3_R, 6_R, 3_C, -3_C, 6_C, K or K8.  Writes at most cap bytes. */
 int tw_result_folder(const tw_input* in, const tw_config* cfg, const char* root, char* out, int32_t cap);
 
 /* Data misfit of the recorded traces against data [n][nt] (the solver's

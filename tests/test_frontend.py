@@ -170,6 +170,8 @@ def test_result_folder(tmp_path):
         "result/elastic/homogeneous/device/grid_size_1_125/Nt_60000/fp32/dt_1p000000000000000e-04/6_R"
     cfg = FE.input_to_config(inp, precision="bf16", cpst="K")
     assert FE.result_folder(inp, cfg, "r").endswith("/bf16/dt_1p000000000000000e-04/K")
+    cfg8 = FE.input_to_config(inp, precision="bf16", cpst="K8")
+    assert FE.result_folder(inp, cfg8, "r").endswith("/bf16/dt_1p000000000000000e-04/K8")
     # the reference's dt folder: "%16.15e" with the decimal point replaced
     assert "dt_" + ("%16.15e" % cfg.dt).replace(".", "p", 1) in FE.result_folder(inp, cfg, "r")
     small = ctypes.create_string_buffer(8)
