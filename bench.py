@@ -13,8 +13,10 @@ A step is one full leapfrog step (velocity and stress half steps, SAT,
 compensated update, source, receivers, FP64 energy) of all 5 fields.
 value = cell updates per second over the K timed steps, inputs resident in
 HBM.  The same run times the FP32-storage solver on the same workload,
-measures the BF16 error against an FP64 run of the same inputs, and times
-the CPU oracle on a bounded sample of the workload.
+measures the BF16 error against an FP64 run of the same inputs, times and
+checks other storage variants (by default uncompensated BF16 and FP16 with
+8-bit residual codes) on the same material, and times the CPU oracle on a
+bounded sample of the workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size M]
 """
@@ -290,7 +292,10 @@ def main():
                     help="compensation; K8 = the Kahan residual kept as an 8-bit code per value (the mixed-precision "
                     "config), K = 16-bit residual, 0 = none")
     ap.add_argument("--coef16", type=int, default=1, help="store material coefficients in the 16-bit type")
-    ap.add_argument("--no-compare", action="store_true", help="skip the FP32 and FP64 runs")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP32, FP64 and variant runs")
+    ap.add_argument("--variants", default="bf16:0,fp16:K8",
+                    help="comma-separated storage:cpst pairs timed and checked against FP64 beside the main run "
+                    "(16-bit coefficients, same material)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-launch events in the timed region (roofline from the step time)")
@@ -388,6 +393,21 @@ def main():
             "storage": {"fields": max(rl2(fields_lp[f], f64[f]) for f in f64), "receivers": rl2(rec_lp, rec64)},
             "fp32": {"fields": max(rl2(fields32[f], f64[f]) for f in f64), "receivers": rl2(rec32, rec64)},
         }
+        # other storage variants on the same material and steps: speed and error side by side
+        out["variants"] = []
+        for spec in filter(None, args.variants.split(",")):
+            vp, vc = spec.split(":")
+            sv = make_solver(T, W, m, T.PRECISION[vp], vc, 1, nt, model_from=s)
+            tv, _, _ = timed_steps(torch, sv, K, Wn)
+            fv = {f: sv.get_field(f) for f in sv.fields}
+            rv = sv.receivers()
+            sv.close()
+            val = m * m * K / tv
+            out["variants"].append({
+                "storage": vp, "cpst": vc, "coef16": 1, "value": val, "ms_per_step": tv / K * 1e3,
+                "speedup_vs_fp32": val / out["fp32_value"],
+                "bytes_per_update": algorithmic_bytes(T.PRECISION[vp], vc, 1),
+                "rel_l2_vs_fp64": {"fields": max(rl2(fv[f], f64[f]) for f in f64), "receivers": rl2(rv, rec64)}})
     s.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_size or m)

@@ -1,0 +1,208 @@
+"""The benchmark configurations at their full sizes (GPU).
+
+The oracle cannot run thousands of steps on 4096^2 and 8192^2 grids in a test,
+so full sizes are checked two ways:
+
+* a short window against the oracle: the real workload (layered model, source
+  at 100 m depth, 256 surface receivers, free-surface top and characteristic
+  sides) from seeded random initial fields, a few steps, fields, residual codes
+  and receiver traces bit for bit -- every 256-column strip, 16-row chunk and
+  SBP band tile of the full-size launch geometry takes part;
+* size-independent properties over the full step count: a repeated run is
+  bitwise identical, the discrete energy E = E_k + (E_p0 + E_p1)/2 does not
+  grow once the source has stopped (free surface conserves, characteristic
+  sides dissipate), FP32 storage stays within 1e-4 relative L2 of FP64, and
+  the compensated 16-bit variants keep their measured error and stay below
+  the uncompensated one.
+The batched-shots configuration is checked at its full grid sizes by
+bitwise equality of each problem with the same problem stepped alone.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import common as C
+import oracle as O
+import twave as T
+import twave.workloads as W
+
+pytestmark = pytest.mark.gpu
+
+DX_KM = 0.005
+
+
+def elastic_pair(m, precision, cpst, coef16, nt, with_oracle=True):
+    cfg = W.elastic_config(m=m, precision=precision, cpst=cpst, coef16=coef16, nt=nt)
+    s = T.Solver(cfg)
+    W.setup_elastic(s, m, DX_KM)
+    o = None
+    if with_oracle:
+        o = O.Oracle(O.make_config(physics=0, mx=m, my=m, dx=cfg.dx, num_dx=cfg.num_dx, den_dx=cfg.den_dx,
+                                   dt=cfg.dt, bc="FCCC", precision=precision, cpst=cpst, coef16=coef16,
+                                   nt_max=nt))
+        W.setup_elastic(o, m, DX_KM)
+    return s, o
+
+
+def random_state(s, o, seed, amp=1e-3):
+    rng = np.random.default_rng(seed)
+    for f in s.fields:
+        v = amp * rng.standard_normal(s.shape(f))
+        s.set_field(f, v)
+        o.set_field(f, v)
+
+
+@pytest.mark.parametrize("m,precision,cpst,coef16,steps", [
+    (4096, T.FP32, "0", 0, 6),     # plain elastic FP32
+    (8192, T.BF16, "K8", 1, 3),    # mixed precision: BF16 state, 8-bit residual codes, 16-bit coefficients
+])
+def test_full_size_window_bitwise(m, precision, cpst, coef16, steps):
+    print("case full-size window", m, precision, cpst, coef16, steps, flush=True)
+    O.set_threads(16)
+    s, o = elastic_pair(m, precision, cpst, coef16, steps + 2)
+    random_state(s, o, 11)
+    s.step(1)
+    s.step(steps - 1)  # two calls: the carried work crosses a tw_step boundary
+    o.step(steps)
+    C.compare_state(s, o, exact=True, companions=True)
+    np.testing.assert_array_equal(s.receivers(), o.receivers())
+    # energies: the same FP64 terms summed in another order (block tree vs the
+    # oracle's sequential loop); a sequential sum of n positive terms is within
+    # (n - 1) u of exact, here n = every point of every field
+    n = sum(r * c for r, c in (s.shape(f) for f in s.fields))
+    for a, b in zip(s.energy(), o.energy()):
+        np.testing.assert_allclose(a, b, rtol=n * 2.0 ** -53, atol=0)
+    s.close()
+
+
+def run_full(m, precision, cpst, coef16, nt, model_from=None):
+    s, _ = elastic_pair(m, precision, cpst, coef16, nt, with_oracle=False)
+    if model_from is not None:  # the same (quantized) material in every run
+        for k in (T.C_BINV_VX, T.C_BINV_VY, T.C_MU, T.C_LAMBDA, T.C_TWOMU):
+            s.set_coef(k, model_from.get_coef(k))
+    s.step(nt)
+    return s
+
+
+def state(s):
+    return {f: s.get_field(f) for f in s.fields}, s.receivers()
+
+
+def total_energy(s):
+    ek, ep0, ep1 = s.energy()
+    return ek + 0.5 * (ep0 + ep1)
+
+
+def source_off_step(cfg_dt, f0=15.0):
+    """First step after which the Ricker source (t0 = 1.25/f0) is below 1e-6
+    of its peak: |t - t0| > sqrt(ln(1e6)) / (pi f0) plus one period."""
+    t_end = 1.25 / f0 + math.sqrt(math.log(1e6)) / (math.pi * f0) + 1.0 / f0
+    return int(math.ceil(t_end / cfg_dt))
+
+
+def test_c2_elastic_fp32_full():
+    """4096^2, FP32, 2000 steps: determinism, energy after the source stops,
+    FP32 vs FP64 within 1e-4 relative L2 (fields and receiver traces)."""
+    m, nt = 4096, 2000
+    print("case C2 full", m, nt, flush=True)
+    a = run_full(m, T.FP32, "0", 0, nt)
+    b = run_full(m, T.FP32, "0", 0, nt)
+    fa, ra = state(a)
+    fb, rb = state(b)
+    for f in fa:
+        np.testing.assert_array_equal(fa[f], fb[f], err_msg=T.FIELD_NAMES[f])
+    np.testing.assert_array_equal(ra, rb)
+    b.close()
+    E = total_energy(a)
+    n0 = source_off_step(W.elastic_config(m=m, nt=nt).dt)
+    assert n0 < nt - 100
+    tail = E[n0:nt]
+    assert np.all(np.isfinite(tail)) and tail[0] > 0
+    growth = float(np.max(np.diff(tail) / tail[:-1]))
+    print("C2 energy after the source (step %d): %.6e -> %.6e, largest one-step relative growth %.2e"
+          % (n0, tail[0], tail[-1], growth), flush=True)
+    assert tail.max() <= tail[0] * (1 + 1e-5)
+    assert tail[-1] <= tail[0]
+    s64 = run_full(m, T.FP64, "0", 0, nt, model_from=a)
+    f64, r64 = state(s64)
+    s64.close()
+    err_f = max(C.rel_l2(fa[f], f64[f]) for f in f64)
+    err_r = C.rel_l2(ra, r64)
+    print("C2 FP32 vs FP64 rel-L2: fields %.3e receivers %.3e" % (err_f, err_r), flush=True)
+    assert err_f < 1e-4 and err_r < 1e-4
+    a.close()
+
+
+def test_c3_mixed_precision_full():
+    """8192^2, 2000 steps on the same quantized material: the K8 variant
+    (BF16 + 8-bit residual codes) stays within its measured error against
+    FP64 and well below uncompensated BF16; FP16 with residual codes below
+    1e-2; a repeated K8 run is bitwise identical."""
+    m, nt = 8192, 2000
+    print("case C3 full", m, nt, flush=True)
+    k8 = run_full(m, T.BF16, "K8", 1, nt)
+    k8b = run_full(m, T.BF16, "K8", 1, nt, model_from=k8)
+    fk, rk = state(k8)
+    fkb, rkb = state(k8b)
+    k8b.close()
+    for f in fk:
+        np.testing.assert_array_equal(fk[f], fkb[f], err_msg=T.FIELD_NAMES[f])
+    np.testing.assert_array_equal(rk, rkb)
+    del fkb
+    s64 = run_full(m, T.FP64, "0", 0, nt, model_from=k8)
+    f64, r64 = state(s64)
+    s64.close()
+    plain = run_full(m, T.BF16, "0", 1, nt, model_from=k8)
+    fp, rp = state(plain)
+    plain.close()
+    h16 = run_full(m, T.FP16, "K8", 1, nt, model_from=k8)
+    fh, rh = state(h16)
+    h16.close()
+    k8.close()
+    e = {name: (max(C.rel_l2(ff[f], f64[f]) for f in f64), C.rel_l2(rr, r64))
+         for name, ff, rr in (("bf16-K8", fk, rk), ("bf16-0", fp, rp), ("fp16-K8", fh, rh))}
+    print("C3 rel-L2 vs FP64 (fields, receivers):", e, flush=True)
+    assert e["bf16-K8"][0] < 5e-2 and e["bf16-K8"][1] < 5e-2
+    assert e["bf16-K8"][0] < 0.5 * e["bf16-0"][0]
+    assert e["fp16-K8"][0] < 1e-2 and e["fp16-K8"][1] < 1e-2
+
+
+def test_c4_batch_full_sizes():
+    """Batched acoustic shots at the configuration's extreme grid sizes
+    (512^2 and 2048^2, plus one in between), smooth random velocity, Ricker at
+    a random interior point, BF16 storage: every problem of the batch equals
+    the same problem stepped alone, bit for bit."""
+    print("case C4 full sizes", flush=True)
+    sizes = [512, 1300, 2048]
+    dx, vmax = 0.01, 4.5
+
+    def build():
+        rng = np.random.default_rng(4)
+        out = []
+        for i, n in enumerate(sizes):
+            rho, vp, _ = W.smooth_acoustic_model(n, dx, seed=1000 + i)
+            cfg = T.config(physics=T.ACOUSTIC, mx=n, my=n, dx=dx, num_dx=1, den_dx=100,
+                           dt=0.5 * dx / (math.sqrt(2.0) * vmax), bc="CCCC", precision=T.BF16, cpst="0",
+                           coef16=1, nt_max=300, record_energy=0, stencil_form=1)
+            s = T.Solver(cfg)
+            s.set_model(rho, vp, vp, 1, 100)
+            f0 = float(rng.uniform(10.0, 25.0))
+            ix, iy = (int(v) for v in rng.integers(n // 8, n - n // 8, 2))
+            s.set_source_ricker(T.GRID_MM, ix, iy, f0, 1.25 / f0)
+            out.append(s)
+        return out
+
+    batched, solo = build(), build()
+    B = T.Batch(batched)
+    B.step(150)
+    B.step(50)
+    for s in solo:
+        s.step(200)
+    for x, y in zip(batched, solo):
+        for f in x.fields:
+            np.testing.assert_array_equal(x.get_field(f), y.get_field(f), err_msg=T.FIELD_NAMES[f])
+        assert np.any(x.get_field(T.P) != 0)
+    B.close()
+    for s in batched + solo:
+        s.close()

@@ -106,6 +106,13 @@ def test_elastic_wide_k8(bc, precision):
     run_case("elastic", bc, precision, "K8", coef16=1, mx=21, my=531, nt=12)
 
 
+@pytest.mark.parametrize("bc,precision", [("FCCC", "bf16"), ("CCCC", "fp16"), ("FFFF", "bf16")])
+def test_elastic_sbp_k8(bc, precision):
+    """The 8-bit residual with SBP closures on both axes (the fused step's
+    configurations): residual codes written to the other buffer of each pair."""
+    run_case("elastic", bc, precision, "K8", coef16=1, mx=40, my=48, nt=30)
+
+
 @pytest.mark.parametrize("bc", ["PPPP", "CCCC", "PPFC"])
 def test_acoustic_wide(bc):
     run_case("acoustic", bc, "fp32", "0", mx=17, my=517, nt=12, stencil_form=1)
