@@ -15,14 +15,16 @@ import twave as T
 
 pytestmark = pytest.mark.gpu
 
-# Kernel variants: streaming kernels with the default chunk of 16 rows, with a
-# 5-row chunk (many chunk boundaries and window warm-ups), the per-point
+# Kernel variants: streaming kernels with the default chunk height of the
+# storage type (14, 16 or 20 rows), with a 5-row chunk (many chunk boundaries
+# and window warm-ups), with 16-row chunks and the bands in launches of their
+# own ("stream-sepbands"), the per-point
 # kernels, and the fused single-pass step (elastic, SBP on both axes; other
 # configurations fall back to the streaming kernels) with both chunk sizes.
-# The streaming kernels run the SBP band tiles as extra blocks of their core
-# launches; "stream-sepbands" keeps the bands in launches of their own.
-# Read by tw_create from the environment.
-KERNELS = [("2", "16", "1"), ("2", "5", "1"), ("2", "16", "0"), ("1", "16", "1"), ("3", "16", "1"), ("3", "5", "1")]
+# The streaming kernels otherwise run the SBP band tiles as extra blocks of
+# their core launches.  Read by tw_create from the environment (an empty
+# TW_TXC keeps the default).
+KERNELS = [("2", "", "1"), ("2", "5", "1"), ("2", "16", "0"), ("1", "16", "1"), ("3", "16", "1"), ("3", "5", "1")]
 ENV = ("TW_KERNEL", "TW_TXC", "TW_MERGE_BANDS")
 
 

@@ -550,7 +550,9 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   p.energy_form = c->energy_form;
   p.dx = c->dx;
   if (const char* kv = getenv("TW_KERNEL")) p.kver = atoi(kv) == 1 ? 1 : (atoi(kv) == 3 ? 3 : 2);
-  if (const char* tx = getenv("TW_TXC")) p.txc = atoi(tx) > 0 ? atoi(tx) : 16;
+  p.txc = tw::default_chunk_rows(c->precision, c->cpst);
+  if (const char* tx = getenv("TW_TXC"))
+    if (atoi(tx) > 0) p.txc = atoi(tx);
   if (const char* mb = getenv("TW_MERGE_BANDS")) p.merge_bands = atoi(mb) != 0;
   int pad = 0;
   if (const char* pp = getenv("TW_PITCH_PAD")) pad = atoi(pp) > 0 ? (atoi(pp) + 3) / 4 * 4 : 0;

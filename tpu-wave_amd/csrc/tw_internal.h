@@ -62,6 +62,17 @@ struct Pending {
   int32_t rcv_it = 0;
 };
 
+// Rows per chunk of the streaming kernels: each chunk re-reads 3 window rows
+// of the x partners, longer chunks leave fewer blocks to balance the launch.
+// Measured on the 8192^2 elastic workload over chunks of 8-32 rows: 16-bit
+// storage with a residual array is fastest at 13-14 rows, FP32 at 20-24,
+// 16-bit without a residual at 16.
+inline int32_t default_chunk_rows(int precision, int cpst) {
+  const bool sixteen = precision == TW_BF16 || precision == TW_FP16;
+  if (sixteen) return (cpst == TW_CPST_K || cpst == TW_CPST_K8) ? 14 : 16;
+  return precision == TW_FP32 ? 20 : 16;
+}
+
 struct Plan {
   int32_t physics;
   int32_t precision;
@@ -77,7 +88,7 @@ struct Plan {
   void* dstat = nullptr;  // device copy of the static kernel arguments (in place, current buffers)
   void* dstat_ip[2] = {}; // in-place arguments for each parity
   void* dstat_pp[2] = {}; // fused-step arguments (read current, write other) for each parity
-  int32_t txc = 16;   // rows per chunk of the streaming kernels (TW_TXC overrides)
+  int32_t txc = 16;   // rows per chunk of the streaming kernels (default_chunk_rows; TW_TXC overrides)
   int32_t merge_bands = 1;  // band tiles as extra blocks of the core launches (TW_MERGE_BANDS=0: own launch)
   // timing of this step's core launches: start/stop of velocity, stress (null: off)
   hipEvent_t prof[4] = {};
