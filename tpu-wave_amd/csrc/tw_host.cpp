@@ -341,10 +341,6 @@ void dfree(tw_solver* s, void** p) {
   *p = nullptr;
 }
 
-// row pitch in elements: columns rounded up to 64, plus an optional pad
-// (TW_PITCH_PAD, a multiple of 4 elements) that breaks power-of-two strides
-int64_t pitch_of(int cols, int pad) { return ((int64_t)cols + 63) / 64 * 64 + pad; }
-
 int upload_coef(tw_solver* s, int k) {
   const tw_config* c = &s->cfg;
   tw::DevCoef& d = s->plan.k[k];
@@ -566,7 +562,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     if (!field_in(c->physics, f)) continue;
     tw::DevField& d = p.f[f];
     shape_of(c, field_grid(f), &d.rows, &d.cols);
-    d.pitch = pitch_of(d.cols, pad);
+    d.pitch = tw::pitch_of(d.cols, pad);
     const size_t bytes = (size_t)d.rows * d.pitch * es;
     const size_t rbytes = (size_t)d.rows * d.pitch * array_size(c, 1);
     if ((e = dalloc(s, &d.s, bytes))) return bail(e);
@@ -587,7 +583,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     if (!coef_in(c->physics, k)) continue;
     tw::DevCoef& d = p.k[k];
     shape_of(c, coef_grid(k), &d.rows, &d.cols);
-    d.pitch = pitch_of(d.cols, pad);
+    d.pitch = tw::pitch_of(d.cols, pad);
     if ((e = dalloc(s, &d.p, (size_t)d.rows * d.pitch * ks))) return bail(e);
     s->hcoef[k].assign((size_t)d.rows * d.cols, 0.0);
   }

@@ -62,6 +62,11 @@ struct Pending {
   int32_t rcv_it = 0;
 };
 
+// Row pitch in elements: columns rounded up to 128, so every row of every
+// array (16-bit state, and the one-byte residual codes) starts on a 128-byte
+// cache line; pad adds elements (TW_PITCH_PAD, measurements only).
+inline int64_t pitch_of(int cols, int pad) { return ((int64_t)cols + 127) / 128 * 128 + pad; }
+
 // Rows per chunk of the streaming kernels: each chunk re-reads 3 window rows
 // of the x partners, longer chunks leave fewer blocks to balance the launch.
 // Measured on the 8192^2 elastic workload over chunks of 8-32 rows: 16-bit

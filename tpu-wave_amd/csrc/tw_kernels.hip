@@ -969,6 +969,9 @@ __device__ __forceinline__ V4<ST> ld_v4(PT p) {
   return o;
 }
 
+// Stores of new state and residual rows: written once per half step and not
+// read again before the next one, so (below FP64) they are non-temporal
+// streaming stores that do not keep the lines in the caches.
 template <class ST, class PT>
 __device__ __forceinline__ void st_v4(PT p, const V4<ST>& v) {
   if constexpr (sizeof(ST) == 8) {
@@ -983,7 +986,25 @@ __device__ __forceinline__ void st_v4(PT p, const V4<ST>& v) {
     typedef __attribute__((address_space(1))) R* Q;
     R r;
     __builtin_memcpy(&r, &v, sizeof r);
-    *(Q)p = r;
+    __builtin_nontemporal_store(r, (Q)p);
+  }
+}
+
+// Loads of rows that a half step reads exactly once (coefficients, and the
+// old state and residual of the fields it updates): non-temporal below FP64,
+// so they do not displace the partner rows that neighbouring chunks and
+// strips read again (window warm-up rows, halo lines).
+template <class ST, class PT>
+__device__ __forceinline__ V4<ST> ld_v4s(PT p) {
+  if constexpr (sizeof(ST) == 8) {
+    return ld_v4<ST>(p);
+  } else {
+    V4<ST> o;
+    typedef typename RawOf<sizeof(ST)>::v R;
+    typedef __attribute__((address_space(1))) const R* Q;
+    const R r = __builtin_nontemporal_load((Q)p);
+    __builtin_memcpy(&o, &r, sizeof o);
+    return o;
   }
 }
 
@@ -1177,6 +1198,21 @@ __device__ __forceinline__ V4<ST> ld_r4(const FR<ST>& X, int row, int c) {
   }
 }
 
+// ld_r4 for residual rows read once per half step (non-temporal, as ld_v4s)
+template <class ST, int CP>
+__device__ __forceinline__ V4<ST> ld_r4s(const FR<ST>& X, int row, int c) {
+  if constexpr (CP == TW_CPST_K8) {
+    typedef __attribute__((address_space(1))) const uint32_t* Q;
+    const uint32_t w = __builtin_nontemporal_load((Q)(X.brow(row) + c));
+    V4<ST> o;
+#pragma unroll
+    for (int j = 0; j < 4; j++) o.v[j] = __builtin_bit_cast(ST, (int16_t)(int8_t)(w >> (8 * j)));
+    return o;
+  } else {
+    return ld_v4s<ST>(X.rrow(row) + c);
+  }
+}
+
 // store of a lane's four residual values, element-wise where only some columns are in
 template <class ST, int CP>
 __device__ __forceinline__ void st_r4(const FR<ST>& X, int row, int c, const V4<ST>& v, bool full4, const bool* in) {
@@ -1187,7 +1223,7 @@ __device__ __forceinline__ void st_r4(const FR<ST>& X, int row, int c, const V4<
 #pragma unroll
       for (int j = 0; j < 4; j++) w |= ((uint32_t)__builtin_bit_cast(uint16_t, v.v[j]) & 0xffu) << (8 * j);
       typedef __attribute__((address_space(1))) uint32_t* Q;
-      *(Q)p = w;
+      __builtin_nontemporal_store(w, (Q)p);
     } else {
 #pragma unroll
       for (int j = 0; j < 4; j++)
@@ -1660,14 +1696,14 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         oB = ld_v4<ST>(PY.row(ry) + cl);   // syy row r
       }
       const V4<ST> hB = ld_v4<ST>(PY.row(ry) + hc);  // syy / p row r halo
-      const V4<KT> kx = ld_v4<KT>(bxp + (int64_t)rx * bxs + cl);
-      V4<ST> sx = ld_v4<ST>(VX.row(rx) + cl);
+      const V4<KT> kx = ld_v4s<KT>(bxp + (int64_t)rx * bxs + cl);
+      V4<ST> sx = ld_v4s<ST>(VX.row(rx) + cl);
       V4<ST> qx = sx;
-      if constexpr (RM) qx = ld_r4<ST, CP>(VX, rx, cl);
-      const V4<KT> ky = ld_v4<KT>(byp + (int64_t)ry * bys + cl);
-      V4<ST> sy = ld_v4<ST>(VY.row(ry) + cl);
+      if constexpr (RM) qx = ld_r4s<ST, CP>(VX, rx, cl);
+      const V4<KT> ky = ld_v4s<KT>(byp + (int64_t)ry * bys + cl);
+      V4<ST> sy = ld_v4s<ST>(VY.row(ry) + cl);
       V4<ST> qy = sy;
-      if constexpr (RM) qy = ld_r4<ST, CP>(VY, ry, cl);
+      if constexpr (RM) qy = ld_r4s<ST, CP>(VY, ry, cl);
       // ---- segments
       cvt_v4<ST, CT>(nA, a3);
       if constexpr (PH == TW_ELASTIC) {
@@ -1969,25 +2005,25 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       if constexpr (PH == TW_ELASTIC) {
         nA = ld_v4<ST>(VY.row(wrap_row(r + 1, perx, mx)) + cl);
         hA = ld_v4<ST>(VX.row(r) + hc);  // vx row r halo
-        kn = ld_v4<KT>(kmp + (int64_t)r * kms + cl);
-        sn = ld_v4<ST>(XN.row(r) + cl);
+        kn = ld_v4s<KT>(kmp + (int64_t)r * kms + cl);
+        sn = ld_v4s<ST>(XN.row(r) + cl);
         qn = sn;
-        if constexpr (RM) qn = ld_r4<ST, CP>(XN, r, cl);
+        if constexpr (RM) qn = ld_r4s<ST, CP>(XN, r, cl);
       } else {
         oB = ld_v4<ST>(VY.row(rm) + cl);  // vy row r (no window in the acoustic case)
       }
       const V4<ST> nB = ld_v4<ST>(VX.row(wrap_row(r + 2, perx, mx)) + cl);
       const V4<ST> hB = ld_v4<ST>(VY.row(rm) + hc);  // vy row r halo
-      const V4<KT> k0 = ld_v4<KT>(k0p + (int64_t)rm * k0s + cl);
+      const V4<KT> k0 = ld_v4s<KT>(k0p + (int64_t)rm * k0s + cl);
       V4<KT> k1;
-      if constexpr (PH == TW_ELASTIC) k1 = ld_v4<KT>(k1p + (int64_t)rm * k1s + cl);
-      V4<ST> s0 = ld_v4<ST>(X0.row(rm) + cl), s1;
+      if constexpr (PH == TW_ELASTIC) k1 = ld_v4s<KT>(k1p + (int64_t)rm * k1s + cl);
+      V4<ST> s0 = ld_v4s<ST>(X0.row(rm) + cl), s1;
       V4<ST> q0 = s0, q1;
-      if constexpr (RM) q0 = ld_r4<ST, CP>(X0, rm, cl);
+      if constexpr (RM) q0 = ld_r4s<ST, CP>(X0, rm, cl);
       if constexpr (PH == TW_ELASTIC) {
-        s1 = ld_v4<ST>(X1.row(rm) + cl);
+        s1 = ld_v4s<ST>(X1.row(rm) + cl);
         q1 = s1;
-        if constexpr (RM) q1 = ld_r4<ST, CP>(X1, rm, cl);
+        if constexpr (RM) q1 = ld_r4s<ST, CP>(X1, rm, cl);
       }
       // ---- segments
       cvt_v4<ST, CT>(nB, b3);
@@ -2484,10 +2520,10 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
     const V4<ST> hA = ld_v4<ST>(SXY.row(v) + hc);
     sqF = ld_v4<ST>(SYY.row(v) + cl);
     const V4<ST> hB = ld_v4<ST>(SYY.row(v) + hc);
-    const V4<KT> kx = ld_v4<KT>(bxp + (int64_t)v * bxs + cl);
+    const V4<KT> kx = ld_v4s<KT>(bxp + (int64_t)v * bxs + cl);
     V4<ST> sx = ld_v4<ST>(VX.row(v) + cl), qx = sx;
     if constexpr (RM) qx = ld_r4<ST, CP>(VX, v, cl);
-    const V4<KT> ky = ld_v4<KT>(byp + (int64_t)v * bys + cl);
+    const V4<KT> ky = ld_v4s<KT>(byp + (int64_t)v * bys + cl);
     V4<ST> sy = ld_v4<ST>(VY.row(v) + cl), qy = sy;
     if constexpr (RM) qy = ld_r4<ST, CP>(VY, v, cl);
     // stress-row loads: issued with the velocity loads when the variant keeps
@@ -2495,9 +2531,9 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
     V4<KT> km, k0, k1;
     V4<ST> qn, q0, q1;
     auto ld_stress = [&]() {
-      km = ld_v4<KT>(mup + (int64_t)sr * mus + cl);
-      k0 = ld_v4<KT>(lap + (int64_t)sr * las + cl);
-      k1 = ld_v4<KT>(tmp_ + (int64_t)sr * tms + cl);
+      km = ld_v4s<KT>(mup + (int64_t)sr * mus + cl);
+      k0 = ld_v4s<KT>(lap + (int64_t)sr * las + cl);
+      k1 = ld_v4s<KT>(tmp_ + (int64_t)sr * tms + cl);
       if constexpr (RM) {
         qn = ld_r4<ST, CP>(SXY, sr, cl);
         q0 = ld_r4<ST, CP>(SXX, sr, cl);
