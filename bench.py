@@ -6,8 +6,9 @@ perturbation, Vs = Vp/1.73, rho = 310 Vp^0.25), 15 Hz Ricker vertical force at
 100 m depth, free-surface SAT on top, characteristic SAT on the other sides,
 256 surface receivers.  Fields stored in BF16 with the Kahan residual of each
 store kept as an 8-bit code (compensated time-stage sums), FP32 arithmetic,
-material coefficients stored in BF16; the FP32 and FP64 comparison runs use
-the same quantized material.
+material coefficients stored as 8-bit codes of a per-row affine map; the FP32
+and FP64 comparison runs use the same quantized material, and the line reports
+how far that material is from the exact FP64 one (material_max_rel_err).
 
 A step is one full leapfrog step (velocity and stress half steps, SAT,
 compensated update, source, receivers, FP64 energy) of all 5 fields.
@@ -44,14 +45,24 @@ def comp_bytes(s, cpst):
     return s if cpst in ("3R", "-3R", "6R", "3C", "-3C", "6C", "K") else 0
 
 
+def coef_bytes(precision, coef16):
+    """Bytes per stored material coefficient: FP64 with FP64 storage, else
+    FP32, the 16-bit type (coef16 1) or an 8-bit code (coef16 2; the per-row
+    offset and step add 8 bytes per row, under 0.01 B per cell)."""
+    if precision == 0:
+        return 8
+    if precision == 1:
+        return 4
+    return {0: 4, 1: 2, 2: 1}[coef16]
+
+
 def algorithmic_bytes(precision, cpst, coef16, physics=0):
     """Compulsory HBM bytes per cell update: 2 F s (state r+w) + 2 F c
     (stored compensation r+w) + K p (coefficients read)."""
     F, K = (5, 5) if physics == 0 else (3, 3)
     s = {0: 8, 1: 4, 2: 2, 3: 2}[precision]
     c = comp_bytes(s, cpst)
-    p = 8 if precision == 0 else (2 if coef16 else 4)
-    return 2 * F * s + 2 * F * c + K * p
+    return 2 * F * s + 2 * F * c + K * coef_bytes(precision, coef16)
 
 
 def half_bytes(precision, cpst, coef16, half, physics=0):
@@ -63,8 +74,7 @@ def half_bytes(precision, cpst, coef16, half, physics=0):
     F = 2 if half == 0 else (3 if physics == 0 else 1)
     s = {0: 8, 1: 4, 2: 2, 3: 2}[precision]
     c = comp_bytes(s, cpst)
-    p = 8 if precision == 0 else (2 if coef16 else 4)
-    return 2 * F * s + 2 * F * c + F * p
+    return 2 * F * s + 2 * F * c + F * coef_bytes(precision, coef16)
 
 
 def kernel_source_hash():
@@ -120,9 +130,18 @@ def make_solver(T, W, m, precision, cpst, coef16, nt, model_from=None):
     cfg = W.elastic_config(m=m, precision=precision, cpst=cpst, coef16=coef16, nt=nt)
     s = T.Solver(cfg)
     W.setup_elastic(s, m, 0.005)
+    s.material_err = None
     if model_from is not None:  # same (quantized) coefficients as the low-precision run
+        err = 0.0
         for k in (T.C_BINV_VX, T.C_BINV_VY, T.C_MU, T.C_LAMBDA, T.C_TWOMU):
-            s.set_coef(k, model_from.get_coef(k))
+            q = model_from.get_coef(k)
+            if precision == T.FP64:  # its own coefficients are the exact FP64 interpolation
+                own = s.get_coef(k)
+                err = max(err, float(np.max(np.abs(q - own) / np.abs(own))))
+                del own
+            s.set_coef(k, q)
+        if precision == T.FP64:
+            s.material_err = err
     return s
 
 
@@ -253,7 +272,7 @@ def c4_batch(args):
         rho, vp, _ = W.smooth_acoustic_model(n, dx, seed=1000 + i)
         cfg = T.config(physics=T.ACOUSTIC, mx=n, my=n, dx=dx, num_dx=1, den_dx=100,
                        dt=0.5 * dx / (math.sqrt(2.0) * vmax), bc="CCCC", precision=T.PRECISION[args.precision],
-                       cpst="0", coef16=args.coef16, nt_max=K + 64, record_energy=0, stencil_form=1)
+                       cpst="0", coef16=args.c4_coef16, nt_max=K + 64, record_energy=0, stencil_form=1)
         s = T.Solver(cfg)
         s.set_model(rho, vp, vp, 1, 100)
         f0 = float(rng.uniform(10.0, 25.0))
@@ -267,13 +286,13 @@ def c4_batch(args):
     cells = sum(n * n for n in sizes)
     value = cells * K / (ms / 1e3)
     prec = T.PRECISION[args.precision]
-    Bu = algorithmic_bytes(prec, "0", args.coef16, physics=1)
+    Bu = algorithmic_bytes(prec, "0", args.c4_coef16, physics=1)
     out = {"metric": "grid-pt updates/s (batched acoustic shots)", "value": value, "unit": "grid-point updates/s",
            "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": ms / K, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "64 acoustic shots, n x n with n uniform in [512, 2048], dx 10 m, smooth random "
                                   "velocity 1.5-4.5 km/s, Ricker 10-25 Hz, CCCC SAT, storage %s, coef16 %d" %
-                                  (args.precision, args.coef16),
+                                  (args.precision, args.c4_coef16),
                       "problems": nprob, "cells": cells, "tile_rows": rows},
            "roofline": {"bound": "hbm", "achieved": value * Bu / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": value * Bu / 1e9 / HBM_PEAK_GBS, "traffic": None, "bytes_per_update": Bu,
@@ -291,11 +310,13 @@ def main():
     ap.add_argument("--cpst", default="K8",
                     help="compensation; K8 = the Kahan residual kept as an 8-bit code per value (the mixed-precision "
                     "config), K = 16-bit residual, 0 = none")
-    ap.add_argument("--coef16", type=int, default=1, help="store material coefficients in the 16-bit type")
+    ap.add_argument("--coef16", type=int, default=2,
+                    help="material coefficient storage: 0 FP32, 1 the 16-bit type, 2 one byte per value with a "
+                    "per-row affine map (default)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32, FP64 and variant runs")
-    ap.add_argument("--variants", default="bf16:0,fp16:K8",
-                    help="comma-separated storage:cpst pairs timed and checked against FP64 beside the main run "
-                    "(16-bit coefficients, same material)")
+    ap.add_argument("--variants", default="bf16:K8:1,bf16:0:2,fp16:K8:2",
+                    help="comma-separated storage:cpst[:coef16] triples (coef16 default 1) timed and checked "
+                    "against FP64 beside the main run, on the same material")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-launch events in the timed region (roofline from the step time)")
@@ -306,6 +327,8 @@ def main():
                     help="elastic: the mixed-precision elastic case (default); c4: batched acoustic shots")
     ap.add_argument("--shots", type=int, default=64, help="c4: number of problems")
     ap.add_argument("--c4-steps", type=int, default=1500, help="c4: time steps")
+    ap.add_argument("--c4-coef16", type=int, default=1,
+                    help="c4: coefficient storage (16-bit by default: the acoustic batch runs no faster with 8-bit)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (the slab decomposition "
@@ -382,6 +405,7 @@ def main():
         rec32 = s32.receivers()
         s32.close()
         s64 = make_solver(T, W, m, T.FP64, "0", 0, nt, model_from=s)
+        out["material_max_rel_err"] = s64.material_err
         s64.step(nt)
         f64 = {f: s64.get_field(f) for f in s64.fields}
         rec64 = s64.receivers()
@@ -396,17 +420,18 @@ def main():
         # other storage variants on the same material and steps: speed and error side by side
         out["variants"] = []
         for spec in filter(None, args.variants.split(",")):
-            vp, vc = spec.split(":")
-            sv = make_solver(T, W, m, T.PRECISION[vp], vc, 1, nt, model_from=s)
+            vp, vc, vk = (spec.split(":") + ["1"])[:3]
+            vk = int(vk)
+            sv = make_solver(T, W, m, T.PRECISION[vp], vc, vk, nt, model_from=s)
             tv, _, _ = timed_steps(torch, sv, K, Wn)
             fv = {f: sv.get_field(f) for f in sv.fields}
             rv = sv.receivers()
             sv.close()
             val = m * m * K / tv
             out["variants"].append({
-                "storage": vp, "cpst": vc, "coef16": 1, "value": val, "ms_per_step": tv / K * 1e3,
+                "storage": vp, "cpst": vc, "coef16": vk, "value": val, "ms_per_step": tv / K * 1e3,
                 "speedup_vs_fp32": val / out["fp32_value"],
-                "bytes_per_update": algorithmic_bytes(T.PRECISION[vp], vc, 1),
+                "bytes_per_update": algorithmic_bytes(T.PRECISION[vp], vc, vk),
                 "rel_l2_vs_fp64": {"fields": max(rl2(fv[f], f64[f]) for f in f64), "receivers": rl2(rv, rec64)}})
     s.close()
     if not args.no_cpu:

@@ -88,7 +88,12 @@ typedef struct tw_config {
                              'C' characteristic (absorbing) SAT                      */
   int32_t precision;      /* field storage type (tw_precision); arithmetic is FP64
                              for TW_FP64 and FP32 otherwise                          */
-  int32_t coef16;         /* BF16/FP16 only: store coefficients in the 16-bit type  */
+  int32_t coef16;         /* BF16/FP16 only: material coefficient storage.
+                             0: FP32; 1: the 16-bit storage type; 2: one byte
+                             per value, each row mapped affinely onto codes
+                             0..255 (FP32 offset = row minimum, step = row
+                             range / 255, value = offset + code * step in FP32).
+                             tw_get_coef returns the values the kernels use.   */
   int32_t cpst;           /* tw_cpst                                                 */
   int32_t stencil_form;   /* 0: stencil*(dt/dx) (cuda/src/cpp/grid.hpp:281-282);
                              1: stencil*dt/dx (singlecore/src/cpp/namespaces.hpp:607-608) */

@@ -14,9 +14,9 @@ SHAPES = [(40, 48, "FCCC"), (64, 96, "CCCC"), (33, 70, "PPPP"), (80, 37, "FFPP")
           (48, 264, "PPCC")]
 
 
-def problem(physics, precision, cpst, mx, my, bc, seed, nt=200):
+def problem(physics, precision, cpst, mx, my, bc, seed, nt=200, coef16=0):
     kw = C.cfg_kwargs(physics=physics, mx=mx, my=my, bc=bc, precision=precision, cpst=cpst, nt=nt,
-                      stencil_form=1 if physics == "acoustic" else 0)
+                      stencil_form=1 if physics == "acoustic" else 0, coef16=coef16)
     s = T.Solver(T.config(**kw))
     rng = np.random.default_rng(seed)
     ks = [T.C_BINV_VX, T.C_BINV_VY, T.C_KAPPA] if physics == "acoustic" else \
@@ -31,9 +31,9 @@ def problem(physics, precision, cpst, mx, my, bc, seed, nt=200):
     return s
 
 
-def pairs(physics, precision, cpst, shapes=SHAPES):
-    a = [problem(physics, precision, cpst, *sh, seed=i) for i, sh in enumerate(shapes)]
-    b = [problem(physics, precision, cpst, *sh, seed=i) for i, sh in enumerate(shapes)]
+def pairs(physics, precision, cpst, shapes=SHAPES, coef16=0):
+    a = [problem(physics, precision, cpst, *sh, seed=i, coef16=coef16) for i, sh in enumerate(shapes)]
+    b = [problem(physics, precision, cpst, *sh, seed=i, coef16=coef16) for i, sh in enumerate(shapes)]
     return a, b
 
 
@@ -55,6 +55,20 @@ def test_batch_equals_solo_acoustic(precision, cpst):
         s.step(30)
     assert_same(batched, solo)
     assert all(s.current_step() == 30 for s in batched)
+
+
+@pytest.mark.parametrize("physics,precision,cpst", [("acoustic", "bf16", "0"), ("acoustic", "fp16", "K8"),
+                                                    ("elastic", "bf16", "K8")])
+def test_batch_equals_solo_coef8(physics, precision, cpst):
+    """8-bit coefficients (per-row affine codes) in the batched launches."""
+    print("case batch coef8", physics, precision, cpst, flush=True)
+    shapes = SHAPES if physics == "acoustic" else [(40, 48, "FFPP"), (33, 70, "PPPP"), (64, 300, "FFFF")]
+    batched, solo = pairs(physics, precision, cpst, shapes, coef16=2)
+    B = T.Batch(batched)
+    B.step(20)
+    for s in solo:
+        s.step(20)
+    assert_same(batched, solo)
 
 
 def test_batch_equals_solo_elastic():

@@ -38,10 +38,14 @@ def elastic_pair(m, precision, cpst, coef16, nt, with_oracle=True):
     W.setup_elastic(s, m, DX_KM)
     o = None
     if with_oracle:
+        # 8-bit coefficients: the oracle takes the solver's decoded values as FP32
         o = O.Oracle(O.make_config(physics=0, mx=m, my=m, dx=cfg.dx, num_dx=cfg.num_dx, den_dx=cfg.den_dx,
-                                   dt=cfg.dt, bc="FCCC", precision=precision, cpst=cpst, coef16=coef16,
-                                   nt_max=nt))
+                                   dt=cfg.dt, bc="FCCC", precision=precision, cpst=cpst,
+                                   coef16=0 if coef16 == 2 else coef16, nt_max=nt))
         W.setup_elastic(o, m, DX_KM)
+        if coef16 == 2:
+            for k in (T.C_BINV_VX, T.C_BINV_VY, T.C_MU, T.C_LAMBDA, T.C_TWOMU):
+                o.set_coef(k, s.get_coef(k))
     return s, o
 
 
@@ -56,6 +60,7 @@ def random_state(s, o, seed, amp=1e-3):
 @pytest.mark.parametrize("m,precision,cpst,coef16,steps", [
     (4096, T.FP32, "0", 0, 6),     # plain elastic FP32
     (8192, T.BF16, "K8", 1, 3),    # mixed precision: BF16 state, 8-bit residual codes, 16-bit coefficients
+    (8192, T.BF16, "K8", 2, 3),    # the same with 8-bit coefficients (per-row affine codes)
 ])
 def test_full_size_window_bitwise(m, precision, cpst, coef16, steps):
     print("case full-size window", m, precision, cpst, coef16, steps, flush=True)
@@ -162,10 +167,20 @@ def test_c3_mixed_precision_full():
     k8.close()
     e = {name: (max(C.rel_l2(ff[f], f64[f]) for f in f64), C.rel_l2(rr, r64))
          for name, ff, rr in (("bf16-K8", fk, rk), ("bf16-0", fp, rp), ("fp16-K8", fh, rh))}
+    del fp, fh
+    # 8-bit coefficients: its own FP64 twin on the same decoded material
+    c8 = run_full(m, T.BF16, "K8", 2, nt)
+    fc, rc = state(c8)
+    t64 = run_full(m, T.FP64, "0", 0, nt, model_from=c8)
+    g64, q64 = state(t64)
+    t64.close()
+    c8.close()
+    e["bf16-K8-coef8"] = (max(C.rel_l2(fc[f], g64[f]) for f in g64), C.rel_l2(rc, q64))
     print("C3 rel-L2 vs FP64 (fields, receivers):", e, flush=True)
     assert e["bf16-K8"][0] < 5e-2 and e["bf16-K8"][1] < 5e-2
     assert e["bf16-K8"][0] < 0.5 * e["bf16-0"][0]
     assert e["fp16-K8"][0] < 1e-2 and e["fp16-K8"][1] < 1e-2
+    assert e["bf16-K8-coef8"][0] < 5e-2 and e["bf16-K8-coef8"][1] < 5e-2
 
 
 def test_c4_batch_full_sizes():

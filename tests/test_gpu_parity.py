@@ -115,6 +115,52 @@ def test_elastic_sbp_k8(bc, precision):
     run_case("elastic", bc, precision, "K8", coef16=1, mx=40, my=48, nt=30)
 
 
+@pytest.mark.parametrize("physics,bc,precision,cpst,mx,my", [
+    ("elastic", "FCCC", "bf16", "K8", 40, 48), ("elastic", "FFPP", "fp16", "K8", 40, 48),
+    ("elastic", "PPPP", "bf16", "0", 40, 48), ("elastic", "FCFC", "bf16", "K8", 21, 531),
+    ("acoustic", "CCCC", "bf16", "0", 40, 48), ("acoustic", "FCPP", "fp16", "K8", 40, 48),
+    ("acoustic", "PPCC", "bf16", "K", 17, 517)])
+def test_coef8(physics, bc, precision, cpst, mx, my):
+    """8-bit material coefficients (coef16 = 2): the solver quantizes each row
+    onto 256 codes of an affine map; the oracle runs FP32 coefficients set to
+    the values the solver reports (tw_get_coef), which are what its kernels
+    decode, so fields and traces agree bit for bit.  The quantization itself:
+    every value within half a step (row range / 255) of what was set, at most
+    256 distinct values per row."""
+    print("case coef8", physics, bc, precision, cpst, mx, my, flush=True)
+    import oracle as O
+    ac = physics == "acoustic"
+    kw = C.cfg_kwargs(physics=physics, mx=mx, my=my, bc=bc, precision=precision, cpst=cpst, nt=30,
+                      stencil_form=1 if ac else 0)
+    s = T.Solver(T.config(**dict(kw, coef16=2)))
+    o = O.Oracle(O.make_config(**dict(kw, coef16=0)))
+    rng = np.random.default_rng(7)
+    ks = [T.C_BINV_VX, T.C_BINV_VY, T.C_KAPPA] if ac else [T.C_BINV_VX, T.C_BINV_VY, T.C_MU, T.C_LAMBDA, T.C_TWOMU]
+    for k in ks:
+        shape = s.get_coef(k).shape
+        lo, hi = (1.5, 2.5) if k == T.C_LAMBDA else (0.7, 1.3)
+        v = rng.uniform(lo, hi, shape)
+        s.set_coef(k, v)
+        d = s.get_coef(k)
+        step = (v.max(axis=1) - v.min(axis=1)) / 255.0
+        assert np.all(np.abs(d - v) <= 0.5 * step[:, None] * (1 + 1e-5) + 1e-6 * np.abs(v))
+        assert all(len(np.unique(row)) <= 256 for row in d)
+        o.set_coef(k, d)
+    C.random_fields(s, o, 8)
+    C.receivers_all(s, o, 9)
+    grid = T.GRID_MM if ac else T.GRID_NM
+    for h in (s, o):
+        h.set_source_ricker(grid, mx // 2, my // 3, 25.0, 0.01)
+    s.step(13)
+    s.step(17)
+    o.step(30)
+    C.compare_state(s, o, exact=True, companions=True)
+    np.testing.assert_array_equal(s.receivers(), o.receivers())
+    for a, b in zip(s.energy(), o.energy()):
+        np.testing.assert_allclose(a, b, rtol=ENERGY_RTOL, atol=0)
+    s.close()
+
+
 @pytest.mark.parametrize("bc", ["PPPP", "CCCC", "PPFC"])
 def test_acoustic_wide(bc):
     run_case("acoustic", bc, "fp32", "0", mx=17, my=517, nt=12, stencil_form=1)

@@ -114,7 +114,7 @@ size_t array_size(const tw_config* c, int which) {
 size_t coef_size(int precision, int coef16) {
   if (precision == TW_FP64) return 8;
   if (precision == TW_FP32) return 4;
-  return coef16 ? 2 : 4;
+  return coef16 == 2 ? 1 : (coef16 ? 2 : 4);
 }
 size_t arith_size(int precision) { return precision == TW_FP64 ? 8 : 4; }
 
@@ -149,6 +149,7 @@ void from_storage(int precision, const void* in, double* out, size_t n) {
 }
 // coefficient rounding: its storage type (the reference casts to the solver precision)
 double round_coef(const tw_config* c, double v) {
+  if (c->coef16 == 2 && !ref_precision(c->precision)) return v;  // quantized per row at upload
   switch (c->precision) {
     case TW_FP64: return v;
     case TW_FP32: return (double)(float)v;
@@ -183,6 +184,9 @@ int check_config(const tw_config* c, std::string* err) {
   if (c->energy_form == 1 && (c->physics != TW_ACOUSTIC || c->bc[0] != 'P' || c->bc[2] != 'P'))
     return fail(err, TW_EINVAL, "energy_form 1 is the periodic acoustic reference form");
   if (c->energy_form != 0 && c->energy_form != 1) return fail(err, TW_EINVAL, "energy_form");
+  if (c->coef16 < 0 || c->coef16 > 2) return fail(err, TW_EINVAL, "coef16 must be 0, 1 or 2");
+  if (c->coef16 == 2 && ref_precision(c->precision))
+    return fail(err, TW_EINVAL, "8-bit coefficients (coef16 2) need 16-bit storage");
   if (c->nt_max < 0) return fail(err, TW_EINVAL, "nt_max must be >= 0");
   if (!(c->source_scaling > 0)) return fail(err, TW_EINVAL, "source_scaling must be > 0");
   return TW_OK;
@@ -341,9 +345,50 @@ void dfree(tw_solver* s, void** p) {
   *p = nullptr;
 }
 
+// 8-bit coefficients: each row is mapped affinely onto [0, 255] (offset =
+// the row minimum, step = range / 255, both FP32), every value takes the code
+// whose FP32 decode off + q * step is nearest, and the host copy is replaced
+// by the decoded values, which is what the kernels compute with.
+void quantize_row8(double* v, int n, uint8_t* q, float* off, float* step) {
+  double lo = v[0], hi = v[0];
+  for (int i = 1; i < n; i++) {
+    lo = std::min(lo, v[i]);
+    hi = std::max(hi, v[i]);
+  }
+  const float o = (float)lo;
+  const float st = (float)((hi - lo) / 255.0);
+  for (int i = 0; i < n; i++) {
+    int c0 = st > 0 ? (int)std::lround((v[i] - (double)o) / (double)st) : 0;
+    int best = 0;
+    double err = 0.0;
+    for (int c = c0 - 1; c <= c0 + 1; c++) {
+      const int cc = std::min(255, std::max(0, c));
+      const double e = std::fabs((double)tw::c8_dec((unsigned)cc, o, st) - v[i]);
+      if (c == c0 - 1 || e < err) {
+        best = cc;
+        err = e;
+      }
+    }
+    q[i] = (uint8_t)best;
+    v[i] = (double)tw::c8_dec((unsigned)best, o, st);
+  }
+  *off = o;
+  *step = st;
+}
+
 int upload_coef(tw_solver* s, int k) {
   const tw_config* c = &s->cfg;
   tw::DevCoef& d = s->plan.k[k];
+  if (c->coef16 == 2 && !ref_precision(c->precision)) {
+    std::vector<uint8_t> codes((size_t)d.rows * d.pitch, 0);
+    std::vector<float> rp((size_t)d.rows * 2);
+    for (int r = 0; r < d.rows; r++)
+      quantize_row8(s->hcoef[k].data() + (size_t)r * d.cols, d.cols, codes.data() + (size_t)r * d.pitch, &rp[2 * r],
+                    &rp[2 * r + 1]);
+    HIPCHK(s, hipMemcpy(d.p, codes.data(), codes.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.rp, rp.data(), rp.size() * sizeof(float), hipMemcpyHostToDevice));
+    return TW_OK;
+  }
   const int cp = coef_store_precision(c);
   const size_t es = state_size(cp);
   std::vector<uint8_t> buf((size_t)d.rows * d.pitch * es, 0);
@@ -585,6 +630,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     shape_of(c, coef_grid(k), &d.rows, &d.cols);
     d.pitch = tw::pitch_of(d.cols, pad);
     if ((e = dalloc(s, &d.p, (size_t)d.rows * d.pitch * ks))) return bail(e);
+    if (c->coef16 == 2 && (e = dalloc(s, &d.rp, (size_t)d.rows * 2 * sizeof(float)))) return bail(e);
     s->hcoef[k].assign((size_t)d.rows * d.cols, 0.0);
   }
   // characteristic SAT buffers

@@ -25,7 +25,8 @@ struct DevField {
 };
 
 struct DevCoef {
-  void* p = nullptr;  // coefficient storage type (FP64, FP32, or 16-bit)
+  void* p = nullptr;  // coefficient storage type (FP64, FP32, 16-bit, or 8-bit codes)
+  void* rp = nullptr; // 8-bit codes: FP32 offset and step of each row's affine map
   int32_t rows = 0, cols = 0;
   int64_t pitch = 0;
 };
@@ -61,6 +62,14 @@ struct Pending {
   int32_t nrcv = 0;
   int32_t rcv_it = 0;
 };
+
+// Decode of an 8-bit material coefficient: code q of a row whose affine map
+// has FP32 offset and step; two FP32 roundings (no fused multiply-add), the
+// same on the host and in the kernels.
+__host__ __device__ inline float c8_dec(unsigned q, float off, float step) {
+  const float t = (float)q * step;
+  return off + t;
+}
 
 // Row pitch in elements: columns rounded up to 128, so every row of every
 // array (16-bit state, and the one-byte residual codes) starts on a 128-byte

@@ -33,6 +33,14 @@ namespace tw {
 struct bf16 {
   uint16_t u;
 };
+// 8-bit material coefficient: a code q in [0, 255] of its row's affine map
+// value = off + q * step (FP32, two roundings); c8f is a decoded value
+struct c8 {
+  uint8_t q;
+};
+struct c8f {
+  float f;
+};
 
 template <class ST>
 struct St;
@@ -62,14 +70,19 @@ struct St<bf16> {
   }
 };
 template <>
+struct St<c8> {
+  using C = float;
+  static __device__ __forceinline__ float ld(c8f v) { return v.f; }
+};
+template <>
 struct St<_Float16> {
   using C = float;
   static __device__ __forceinline__ float ld(_Float16 v) { return (float)v; }
   static __device__ __forceinline__ _Float16 st(float f) { return (_Float16)f; }
 };
 
-template <class KT, class CT>
-__device__ __forceinline__ CT kld(KT v) {
+template <class KT, class CT, class V>
+__device__ __forceinline__ CT kld(V v) {
   return (CT)St<KT>::ld(v);
 }
 
@@ -149,7 +162,13 @@ template <class KT>
 struct K {
   const KT* p;
   int64_t pitch;
-  __device__ __forceinline__ KT get(int ix, int iy) const { return p[(int64_t)ix * pitch + iy]; }
+  const float* rp;  // 8-bit codes: offset and step of each row (rp[2 r], rp[2 r + 1])
+  __device__ __forceinline__ auto get(int ix, int iy) const {
+    if constexpr (std::is_same<KT, c8>::value)
+      return c8f{c8_dec(p[(int64_t)ix * pitch + iy].q, rp[2 * ix], rp[2 * ix + 1])};
+    else
+      return p[(int64_t)ix * pitch + iy];
+  }
 };
 
 // Solver description that does not change between launches.  The streaming
@@ -187,6 +206,7 @@ static Args<ST, KT, CT> make_args(const Plan& p, const StepCtl* c, double* part)
     a.f[i].pitch = p.f[i].pitch;
     a.k[i].p = (const KT*)p.k[i].p;
     a.k[i].pitch = p.k[i].pitch;
+    a.k[i].rp = (const float*)p.k[i].rp;
   }
   for (int i = 0; i < 4; i++) a.t.st[i] = (CT)p.t.st[i];
   for (int i = 0; i < 4; i++)
@@ -935,6 +955,11 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 template <int B>
 struct RawOf;
 template <>
+struct RawOf<1> {
+  using e = uint8_t;
+  using v = uint32_t;
+};
+template <>
 struct RawOf<2> {
   using e = uint16_t;
   using v = u32x2;
@@ -1452,6 +1477,37 @@ __device__ __forceinline__ void cvt_v4(const V4<ST>& v, CT (&o)[4]) {
   for (int j = 0; j < 4; j++) o[j] = (CT)St<ST>::ld(v.v[j]);
 }
 
+// Four coefficients of a row as loaded (codes, plus the row's affine map for
+// 8-bit codes), converted later; coefficient rows are read once per half step
+template <class KT>
+struct Q4 {
+  V4<KT> v;
+  float off, step;
+};
+template <class KT>
+__device__ __forceinline__ Q4<KT> kq4(const K<KT>& k, int row, int c) {
+  Q4<KT> o;
+  o.v = ld_v4s<KT>(k.p + (int64_t)row * k.pitch + c);
+  if constexpr (std::is_same<KT, c8>::value) {
+    o.off = k.rp[2 * row];
+    o.step = k.rp[2 * row + 1];
+  }
+  return o;
+}
+template <class KT, class CT>
+__device__ __forceinline__ void kcvt4(const Q4<KT>& k, CT (&o)[4]) {
+  if constexpr (std::is_same<KT, c8>::value) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) o[j] = c8_dec(k.v.v[j].q, k.off, k.step);
+  } else {
+    cvt_v4<KT, CT>(k.v, o);
+  }
+}
+template <class KT, class CT>
+__device__ __forceinline__ void kld4(const K<KT>& k, int row, int c, CT (&o)[4]) {
+  kcvt4<KT, CT>(kq4<KT>(k, row, c), o);
+}
+
 // four converted values into a segment at p
 template <class ST, class CT>
 __device__ __forceinline__ void seg_put4(CT* p, const V4<ST>& v) {
@@ -1554,10 +1610,8 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
   PX.load(A.f[PH == TW_ELASTIC ? TW_SXX : TW_P]);
   PY.load(A.f[PH == TW_ELASTIC ? TW_SYY : TW_P]);
   if constexpr (PH == TW_ELASTIC) SXY.load(A.f[TW_SXY]);
-  const KT* bxp = A.k[TW_C_BINV_VX].p;
-  const int64_t bxs = A.k[TW_C_BINV_VX].pitch;
-  const KT* byp = A.k[TW_C_BINV_VY].p;
-  const int64_t bys = A.k[TW_C_BINV_VY].pitch;
+  const K<KT> KBX = A.k[TW_C_BINV_VX];
+  const K<KT> KBY = A.k[TW_C_BINV_VY];
   CT st[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) st[k] = A.t.st[k];
@@ -1612,7 +1666,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       __syncthreads();
       if (anyx && r < VX.rows) {  // ---- vx (NM)
         CT b[4];
-        ld4<KT, CT>(bxp + (int64_t)r * bxs, c, b);
+        kld4<KT, CT>(KBX, r, c, b);
         V4<ST> sn = ld_v4<ST>(VX.row(r) + c);
         V4<ST> rn = sn;
         if constexpr (RM) rn = ld_r4<ST, CP>(VX, r, c);
@@ -1638,7 +1692,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       }
       if (anyy && vy_row) {  // ---- vy (MN)
         CT b[4];
-        ld4<KT, CT>(byp + (int64_t)r * bys, c, b);
+        kld4<KT, CT>(KBY, r, c, b);
         V4<ST> sn = ld_v4<ST>(VY.row(r) + c);
         V4<ST> rn = sn;
         if constexpr (RM) rn = ld_r4<ST, CP>(VY, r, c);
@@ -1701,11 +1755,11 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         oB = ld_v4<ST>(PY.row(ry) + cl);   // syy row r
       }
       const V4<ST> hB = ld_v4<ST>(PY.row(ry) + hc);  // syy / p row r halo
-      const V4<KT> kx = ld_v4s<KT>(bxp + (int64_t)rx * bxs + cl);
+      const Q4<KT> kx = kq4<KT>(KBX, rx, cl);
       V4<ST> sx = ld_v4s<ST>(VX.row(rx) + cl);
       V4<ST> qx = sx;
       if constexpr (RM) qx = ld_r4s<ST, CP>(VX, rx, cl);
-      const V4<KT> ky = ld_v4s<KT>(byp + (int64_t)ry * bys + cl);
+      const Q4<KT> ky = kq4<KT>(KBY, ry, cl);
       V4<ST> sy = ld_v4s<ST>(VY.row(ry) + cl);
       V4<ST> qy = sy;
       if constexpr (RM) qy = ld_r4s<ST, CP>(VY, ry, cl);
@@ -1731,7 +1785,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       __syncthreads();
       if (vx_row) {  // ---- vx (NM)
         CT b[4];
-        cvt_v4<KT, CT>(kx, b);
+        kcvt4<KT, CT>(kx, b);
         CT ya[12];
         if constexpr (PH == TW_ELASTIC) seg12<CT>(segA[buf], lane, ya);
 #pragma unroll
@@ -1751,7 +1805,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       }
       if (vy_row) {  // ---- vy (MN)
         CT b[4];
-        cvt_v4<KT, CT>(ky, b);
+        kcvt4<KT, CT>(ky, b);
         CT yb[12];
         seg12<CT>(segB[buf], lane, yb);
 #pragma unroll
@@ -1824,12 +1878,9 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
     X1.load(A.f[TW_SYY]);
     XN.load(A.f[TW_SXY]);
   }
-  const KT* k0p = A.k[PH == TW_ELASTIC ? TW_C_LAMBDA : TW_C_KAPPA].p;
-  const int64_t k0s = A.k[PH == TW_ELASTIC ? TW_C_LAMBDA : TW_C_KAPPA].pitch;
-  const KT* k1p = A.k[TW_C_TWOMU].p;
-  const int64_t k1s = A.k[TW_C_TWOMU].pitch;
-  const KT* kmp = A.k[TW_C_MU].p;
-  const int64_t kms = A.k[TW_C_MU].pitch;
+  const K<KT> K0 = A.k[PH == TW_ELASTIC ? TW_C_LAMBDA : TW_C_KAPPA];
+  const K<KT> K1 = A.k[TW_C_TWOMU];
+  const K<KT> KMU = A.k[TW_C_MU];
   CT st[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) st[k] = A.t.st[k];
@@ -1885,7 +1936,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       if constexpr (PH == TW_ELASTIC) {
         if (anyn && r < XN.rows) {  // ---- sxy (NN)
           CT mu[4];
-          ld4<KT, CT>(kmp + (int64_t)r * kms, c, mu);
+          kld4<KT, CT>(KMU, r, c, mu);
           V4<ST> sn = ld_v4<ST>(XN.row(r) + c);
           V4<ST> rn = sn;
           if constexpr (RM) rn = ld_r4<ST, CP>(XN, r, c);
@@ -1915,8 +1966,8 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         seg12<CT>(segB[buf], lane, yb);
         if constexpr (PH == TW_ELASTIC) {  // ---- sxx, syy (MM)
           CT lam[4], tmu[4];
-          ld4<KT, CT>(k0p + (int64_t)r * k0s, c, lam);
-          ld4<KT, CT>(k1p + (int64_t)r * k1s, c, tmu);
+          kld4<KT, CT>(K0, r, c, lam);
+          kld4<KT, CT>(K1, r, c, tmu);
           V4<ST> n0 = ld_v4<ST>(X0.row(r) + c);
           V4<ST> n1 = ld_v4<ST>(X1.row(r) + c);
           V4<ST> m0 = n0, m1 = n1;
@@ -1953,7 +2004,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           store_f<FULL, ST, CP>(X1, r, c, n1, m1, fullm, inm);
         } else {  // ---- p (MM, acoustic)
           CT kap[4];
-          ld4<KT, CT>(k0p + (int64_t)r * k0s, c, kap);
+          kld4<KT, CT>(K0, r, c, kap);
           V4<ST> sn = ld_v4<ST>(X0.row(r) + c);
           V4<ST> rn = sn;
           if constexpr (RM) rn = ld_r4<ST, CP>(X0, r, c);
@@ -2006,11 +2057,11 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       const int rm = mm_row ? r : X0.rows - 1;
       // ---- loads
       V4<ST> nA, hA, sn, qn, oB;
-      V4<KT> kn;
+      Q4<KT> kn;
       if constexpr (PH == TW_ELASTIC) {
         nA = ld_v4<ST>(VY.row(wrap_row(r + 1, perx, mx)) + cl);
         hA = ld_v4<ST>(VX.row(r) + hc);  // vx row r halo
-        kn = ld_v4s<KT>(kmp + (int64_t)r * kms + cl);
+        kn = kq4<KT>(KMU, r, cl);
         sn = ld_v4s<ST>(XN.row(r) + cl);
         qn = sn;
         if constexpr (RM) qn = ld_r4s<ST, CP>(XN, r, cl);
@@ -2019,9 +2070,9 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       }
       const V4<ST> nB = ld_v4<ST>(VX.row(wrap_row(r + 2, perx, mx)) + cl);
       const V4<ST> hB = ld_v4<ST>(VY.row(rm) + hc);  // vy row r halo
-      const V4<KT> k0 = ld_v4s<KT>(k0p + (int64_t)rm * k0s + cl);
-      V4<KT> k1;
-      if constexpr (PH == TW_ELASTIC) k1 = ld_v4s<KT>(k1p + (int64_t)rm * k1s + cl);
+      const Q4<KT> k0 = kq4<KT>(K0, rm, cl);
+      Q4<KT> k1;
+      if constexpr (PH == TW_ELASTIC) k1 = kq4<KT>(K1, rm, cl);
       V4<ST> s0 = ld_v4s<ST>(X0.row(rm) + cl), s1;
       V4<ST> q0 = s0, q1;
       if constexpr (RM) q0 = ld_r4s<ST, CP>(X0, rm, cl);
@@ -2052,7 +2103,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       __syncthreads();
       if constexpr (PH == TW_ELASTIC) {  // ---- sxy (NN); every core row is an NN row
         CT mu[4];
-        cvt_v4<KT, CT>(kn, mu);
+        kcvt4<KT, CT>(kn, mu);
         CT ya[12];
         seg12<CT>(segA[buf], lane, ya);
 #pragma unroll
@@ -2075,8 +2126,8 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         seg12<CT>(segB[buf], lane, yb);
         if constexpr (PH == TW_ELASTIC) {  // ---- sxx, syy (MM)
           CT lam[4], tmu[4];
-          cvt_v4<KT, CT>(k0, lam);
-          cvt_v4<KT, CT>(k1, tmu);
+          kcvt4<KT, CT>(k0, lam);
+          kcvt4<KT, CT>(k1, tmu);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             CT a0 = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
@@ -2109,7 +2160,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           }
         } else {  // ---- p (MM, acoustic)
           CT kap[4];
-          cvt_v4<KT, CT>(k0, kap);
+          kcvt4<KT, CT>(k0, kap);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             CT acc = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
@@ -2341,9 +2392,11 @@ size_t stat_fill(const Plan& p, void* out, int pp) {
     case TW_FP64: return stat_fill_t<double, double, double>(p, out, pp);
     case TW_FP32: return stat_fill_t<float, float, float>(p, out, pp);
     case TW_BF16:
+      if (p.coef16 == 2) return stat_fill_t<bf16, c8, float>(p, out, pp);
       if (p.coef16) return stat_fill_t<bf16, bf16, float>(p, out, pp);
       return stat_fill_t<bf16, float, float>(p, out, pp);
     default:
+      if (p.coef16 == 2) return stat_fill_t<_Float16, c8, float>(p, out, pp);
       if (p.coef16) return stat_fill_t<_Float16, _Float16, float>(p, out, pp);
       return stat_fill_t<_Float16, float, float>(p, out, pp);
   }
@@ -2373,9 +2426,11 @@ hipError_t launch_half(const Plan& p, int half, const StepCtl& c, hipStream_t s)
     case TW_FP64: return run_half<double, double, double>(p, half, c, s);
     case TW_FP32: return run_half<float, float, float>(p, half, c, s);
     case TW_BF16:
+      if (p.coef16 == 2) return run_half<bf16, c8, float>(p, half, c, s);
       if (p.coef16) return run_half<bf16, bf16, float>(p, half, c, s);
       return run_half<bf16, float, float>(p, half, c, s);
     default:
+      if (p.coef16 == 2) return run_half<_Float16, c8, float>(p, half, c, s);
       if (p.coef16) return run_half<_Float16, _Float16, float>(p, half, c, s);
       return run_half<_Float16, float, float>(p, half, c, s);
   }
@@ -2386,9 +2441,11 @@ static hipError_t misc(const Plan& p, int what, const Pending* pend, hipStream_t
     case TW_FP64: return run_misc<double, double, double>(p, what, pend, s);
     case TW_FP32: return run_misc<float, float, float>(p, what, pend, s);
     case TW_BF16:
+      if (p.coef16 == 2) return run_misc<bf16, c8, float>(p, what, pend, s);
       if (p.coef16) return run_misc<bf16, bf16, float>(p, what, pend, s);
       return run_misc<bf16, float, float>(p, what, pend, s);
     default:
+      if (p.coef16 == 2) return run_misc<_Float16, c8, float>(p, what, pend, s);
       if (p.coef16) return run_misc<_Float16, _Float16, float>(p, what, pend, s);
       return run_misc<_Float16, float, float>(p, what, pend, s);
   }
@@ -2735,6 +2792,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(
 // ---- fused step launches
 bool fused_ok(const Plan& p) {
   return p.physics == TW_ELASTIC && !p.perx && !p.pery && p.energy_form == 0 && p.mx >= 12 && p.my >= 12 &&
+         p.coef16 != 2 &&
          !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C || p.cpst == TW_CPST_K8);
 }
 
@@ -2932,9 +2990,12 @@ hipError_t launch_batch_half(const Plan& p, int half, const BatchCore* core, int
     case TW_FP64: return run_batch_half<double, double, double>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
     case TW_FP32: return run_batch_half<float, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
     case TW_BF16:
+      if (p.coef16 == 2) return run_batch_half<bf16, c8, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
       if (p.coef16) return run_batch_half<bf16, bf16, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
       return run_batch_half<bf16, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
     default:
+      if (p.coef16 == 2)
+        return run_batch_half<_Float16, c8, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
       if (p.coef16)
         return run_batch_half<_Float16, _Float16, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
       return run_batch_half<_Float16, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
