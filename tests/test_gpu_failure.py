@@ -57,3 +57,28 @@ def test_unstable_time_step_is_caught(physics):
     assert (not s.check_finite()) or grown > 1e6, grown
     assert grown > 1e3
     s.close()
+
+
+def test_bad_material_rejected():
+    """Material input is validated on the host before anything reaches the
+    device: non-finite coefficients, non-positive density or velocity."""
+    kw = C.cfg_kwargs(physics="elastic", mx=32, my=40, bc="FCCC", precision="bf16", cpst="K8", coef16=2, nt=4)
+    s = T.Solver(T.config(**kw))
+    v = s.get_coef(T.C_LAMBDA)
+    v[3, 5] = np.nan
+    with pytest.raises(T.TwError):
+        s.set_coef(T.C_LAMBDA, v)
+    n = (33, 41)
+    rho, vp, vs = np.ones(n), np.full(n, 2.0), np.ones(n)
+    rho[0, 0] = -1.0
+    with pytest.raises(T.TwError):
+        s.set_model(rho, vp, vs, 1, 125)
+    rho[0, 0] = 1.0
+    vs[2, 2] = np.inf
+    with pytest.raises(T.TwError):
+        s.set_model(rho, vp, vs, 1, 125)
+    vs[2, 2] = 1.0
+    s.set_model(rho, vp, vs, 1, 125)
+    s.step(4)
+    assert s.check_finite()
+    s.close()

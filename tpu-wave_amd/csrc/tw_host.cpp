@@ -710,7 +710,12 @@ int tw_set_model(tw_solver* s, const double* rho, const double* vp, const double
   if (!s || !rho || !vp || (s->cfg.physics == TW_ELASTIC && !vs)) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
   (void)hipSetDevice(s->cfg.device);
   const tw_config* c = &s->cfg;
+  if (nxm < 1 || nym < 1) return fail(&s->err, TW_EINVAL, "model grid must have at least 2x2 nodes");
   const size_t nm = (size_t)(nxm + 1) * (nym + 1);
+  for (size_t i = 0; i < nm; i++)
+    if (!(rho[i] > 0) || !(vp[i] > 0) || !std::isfinite(rho[i]) || !std::isfinite(vp[i]) ||
+        (c->physics == TW_ELASTIC && (!(vs[i] >= 0) || !std::isfinite(vs[i]))))
+      return fail(&s->err, TW_EINVAL, "model values must be finite, rho and vp positive");
   std::vector<double> a(nm), b(nm);
   for (int k = 0; k < 6; k++) {
     if (!coef_in(c->physics, k)) continue;
@@ -758,6 +763,7 @@ int tw_set_coef(tw_solver* s, int32_t k, const double* v) {
   if (!coef_in(s->cfg.physics, k)) return fail(&s->err, TW_EINVAL, "coefficient not used by this physics");
   (void)hipSetDevice(s->cfg.device);
   for (size_t i = 0; i < s->hcoef[k].size(); i++) {
+    if (!std::isfinite(v[i])) return fail(&s->err, TW_EINVAL, "coefficient must be finite");
     if (!(v[i] > 0) && k != TW_C_LAMBDA) return fail(&s->err, TW_EINVAL, "coefficient must be positive");
     s->hcoef[k][i] = round_coef(&s->cfg, v[i]);
   }
