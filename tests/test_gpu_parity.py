@@ -161,6 +161,38 @@ def test_coef8(physics, bc, precision, cpst, mx, my):
     s.close()
 
 
+@pytest.mark.parametrize("bc,precision,cpst", [("FCCC", "bf16", "K8"), ("CCFF", "fp32", "6R"), ("CCCC", "fp64", "0")])
+def test_characteristic_boundary_values_after_host_changes(bc, precision, cpst):
+    """Velocities changed from the host between tw_step calls (set_field,
+    checkpoint restore) reach the characteristic SAT's boundary values of the
+    next velocity half step: the run stays bitwise equal to the oracle given
+    the same changes, and a restored checkpoint replays bit for bit."""
+    print("case uhat host changes", bc, precision, cpst, flush=True)
+    s, o = C.make_pair(physics="elastic", mx=40, my=48, bc=bc, precision=precision, cpst=cpst, nt=40)
+    C.random_material(s, o, 3)
+    C.random_fields(s, o, 4)
+    C.receivers_all(s, o, 5)
+    s.step(7)
+    o.step(7)
+    rng = np.random.default_rng(6)
+    for f in (T.VX, T.VY):
+        v = 1e-3 * rng.standard_normal(s.shape(f))
+        s.set_field(f, v)
+        o.set_field(f, v)
+    blob = s.checkpoint()
+    s.step(5)
+    o.step(5)
+    C.compare_state(s, o, exact=True, companions=True)
+    # back to the checkpoint: the solver must recompute the boundary values
+    s.restore(blob)
+    s.step(5)
+    C.compare_state(s, o, exact=True, companions=True)
+    s.step(6)
+    o.step(6)
+    C.compare_state(s, o, exact=True, companions=True)
+    s.close()
+
+
 @pytest.mark.parametrize("bc", ["PPPP", "CCCC", "PPFC"])
 def test_acoustic_wide(bc):
     run_case("acoustic", bc, "fp32", "0", mx=17, my=517, nt=12, stencil_form=1)
