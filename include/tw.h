@@ -184,6 +184,9 @@ int tw_clear_source(tw_solver* s);
  * cuda/src/cpp/forward.cpp:104-211). Replaces any previous set. */
 int tw_set_receivers(tw_solver* s, int32_t n, const int32_t* field,
                      const int32_t* ix, const int32_t* iy);
+/* The registered receivers in record order: *n = their count; with non-null
+ * arrays (capacity cap) also their fields and indices (TW_ERANGE if cap < n). */
+int tw_get_receivers(tw_solver* s, int32_t cap, int32_t* n, int32_t* field, int32_t* ix, int32_t* iy);
 
 /* Advance nsteps leapfrog steps: one iteration of the time loop of
  * forward_simulation_periodic_y (cuda/src/cpp/forward_simulation.cpp:30-179) /
@@ -313,7 +316,9 @@ int tw_setup_shot(tw_solver* s, const tw_input* in, const tw_srcrcv_entry* entri
  * (grid letters physical X first; each line "% 6d  " then "% 16.15e  " per
  * field, scaled by source_scaling) and, when energy is recorded,
  * Energy_src_<i>_<grid>_<iy>_<ix>.txt with E = E_k + (E_p0 + E_p1)/2 scaled by
- * source_scaling^2, "%16.15e" per line.  folder must exist. */
+ * source_scaling^2, "%16.15e" per line.  folder must exist.  The solver
+ * must record exactly this shot's receivers (as tw_setup_shot registers
+ * them), otherwise TW_ESTATE. */
 int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* entries, int32_t n, int32_t src_index);
 /* Result folder of the reference's device driver
  * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:755-772):

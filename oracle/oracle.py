@@ -102,6 +102,9 @@ def lib():
         L.or_ricker_increment.argtypes = [cfgp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.or_table.argtypes = [vp, ctypes.c_int, dp]
         L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_update_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, dp, dp, dp]
+        L.or_store16_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, dp, dp, dp]
+        L.or_k8_decode_points.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -112,6 +115,43 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def update_points(precision, cpst, s, r, c=None):
+    """The oracle's per-point leapfrog update (reference variants, FP64/FP32)
+    on independent points: returns the new (S, R, C)."""
+    s = np.array(s, dtype=np.float64).ravel().copy()
+    r = np.array(r, dtype=np.float64).ravel().copy()
+    c = np.zeros_like(s) if c is None else np.array(c, dtype=np.float64).ravel().copy()
+    rc = lib().or_update_points(int(precision), CPST[cpst] if isinstance(cpst, str) else int(cpst), s.size,
+                                _dp(s), _dp(r), _dp(c))
+    if rc:
+        raise ValueError("or_update_points: %d" % rc)
+    return s, r, c
+
+
+def store16_points(precision, cpst, t):
+    """The oracle's 16-bit store of FP32 sums t (BF16/FP16; cpst 0, K or K8):
+    returns (stored value, K residual value or K8 code)."""
+    t = np.array(t, dtype=np.float64).ravel().copy()
+    hi = np.zeros_like(t)
+    res = np.zeros_like(t)
+    rc = lib().or_store16_points(int(precision), CPST[cpst] if isinstance(cpst, str) else int(cpst), t.size,
+                                 _dp(t), _dp(hi), _dp(res))
+    if rc:
+        raise ValueError("or_store16_points: %d" % rc)
+    return hi, res
+
+
+def k8_decode_points(precision, hi, q):
+    """Residual value of K8 codes q stored beside 16-bit values hi."""
+    hi = np.array(hi, dtype=np.float64).ravel().copy()
+    q = np.array(q, dtype=np.float64).ravel().copy()
+    out = np.zeros_like(hi)
+    rc = lib().or_k8_decode_points(int(precision), hi.size, _dp(hi), _dp(q), _dp(out))
+    if rc:
+        raise ValueError("or_k8_decode_points: %d" % rc)
+    return out
 
 
 def set_threads(n):
@@ -243,7 +283,7 @@ class Oracle:
         return out[:n]
 
 
-def acoustic_reference_config(precision=FP64, nt_max=60000, multiple=1):
+def acoustic_reference_config(precision=FP64, nt_max=60000, multiple=1, cpst="0"):
     """The single-core acoustic reference run (singlecore/src/main_cpp/cpu_acoustic.cpp:72-87):
     600*multiple cells per axis, periodic, dx = (1/(5*2.5))/(10*multiple), dt = 1e-4,
     rho = vp = 1."""
@@ -251,5 +291,5 @@ def acoustic_reference_config(precision=FP64, nt_max=60000, multiple=1):
     dx = (1.0 / (5.0 * 2.5)) / ppw
     n = 60 * ppw
     return make_config(physics=ACOUSTIC, mx=n, my=n, dx=dx, num_dx=0, den_dx=0, dt=1e-4,
-                       bc="PPPP", precision=precision, cpst="0", stencil_form=1,
+                       bc="PPPP", precision=precision, cpst=cpst, stencil_form=1,
                        energy_form=1, nt_max=nt_max)

@@ -673,3 +673,27 @@ int or_get_companion(or_solver* o, int f, double* out) {
 }
 
 int or_table(or_solver* o, int which, double* out) { return o->vt->table(o->impl, which, out); }
+
+/* ---- single-point update and residual-code entry points (tests only) ---- */
+int or_update_points(int precision, int cpst, long n, double* s, double* r, double* c) {
+  if (cpst < TW_CPST_0 || cpst > TW_CPST_6C) return TW_EINVAL;
+  if (precision == TW_FP64) f64_update_points(cpst, n, s, r, c);
+  else if (precision == TW_FP32) f32_update_points(cpst, n, s, r, c);
+  else return TW_EINVAL;
+  return 0;
+}
+
+int or_store16_points(int precision, int cpst, long n, const double* t, double* hi, double* res) {
+  if (cpst != TW_CPST_0 && cpst != TW_CPST_K && cpst != TW_CPST_K8) return TW_EINVAL;
+  if (precision == TW_BF16) bf16_store_points(cpst, n, t, hi, res);
+  else if (precision == TW_FP16) f16_store_points(cpst, n, t, hi, res);
+  else return TW_EINVAL;
+  return 0;
+}
+
+int or_k8_decode_points(int precision, long n, const double* hi, const double* q, double* out) {
+  if (precision == TW_BF16) bf16_k8_decode_points(n, hi, q, out);
+  else if (precision == TW_FP16) f16_k8_decode_points(n, hi, q, out);
+  else return TW_EINVAL;
+  return 0;
+}

@@ -65,6 +65,14 @@ double or_ricker_increment(const tw_config* cfg, double f0, double t0, int it, i
  *         6 SAT lift M L[3], 7 SAT lift M R[3], 8 H diag N [4], 9 H diag M [3] */
 int  or_table(or_solver* o, int which, double* out);
 void or_set_threads(int n);
+/* the per-point update the time step applies, on n independent points:
+ * FP64/FP32 reference variants (TW_CPST_0 .. TW_CPST_6C) update (s, r, c) in
+ * place, r being the accumulated right-hand side (values must be exact in the
+ * precision); BF16/FP16 store the FP32 sum t and return the stored value and
+ * the K residual (value) or the K8 code; k8_decode maps codes to residuals */
+int  or_update_points(int precision, int cpst, long n, double* s, double* r, double* c);
+int  or_store16_points(int precision, int cpst, long n, const double* t, double* hi, double* res);
+int  or_k8_decode_points(int precision, long n, const double* hi, const double* q, double* out);
 
 #ifdef __cplusplus
 }

@@ -376,36 +376,65 @@ static void FN(sat)(FN(st_t) * S, int f, int pf, int coefk, int axis, int out_t)
 }
 
 /* ---------- update (grid_update.tpp:1-128, grid_update.cut:28-124) ---------- */
+#if REFMODE
+/* one point of the leapfrog update S <- S + R in the reference's variants.
+ * In: s (state), r (accumulated right-hand side, including the residual the
+ * 'R' variants carried over), c (the 'C' variants' separate compensation).
+ * '0': S = s + r, R = 0 (device_update_00, grid_update.cut:62-67);
+ * 3R: Fast2Sum (s, r) -> (S, R): a = s + r, z = a - s, R = r - z
+ *     (device_fast2sum :28-34 via device_update_3R_fast2sum :113-117;
+ *      sum_3op, singlecore/src/cpp/namespaces.hpp:633-640);
+ * -3R: the same three operations written out (device_update_3R :102-110);
+ * 6R: 2Sum (s, r) -> (S, R): a = s + r, pa = a - r, pb = a - pa,
+ *     R = (s - pa) + (r - pb) (device_slow2sum :42-55 via
+ *     device_update_6R_slow2sum :120-124; sum_6op, namespaces.hpp:643-655);
+ * 3C / -3C / 6C: the same with Y = r + c in place of r, the error into C
+ *     and R = 0 (device_update_3C_fast2sum :83-89, device_update_3C :70-80,
+ *     device_update_6C_slow2sum :92-98).  The dispatch on (cpst_N, cpst_S)
+ *     is grid_update.cut:143-151. */
+static inline void FN(upd_point)(int cpst, SREAL* ps, SREAL* pr, SREAL* pc, CREAL r) {
+  CREAL s = *ps;
+  switch (cpst) {
+    case TW_CPST_0: *ps = s + r; *pr = 0; break;
+    case TW_CPST_3R: { CREAL a = s + r; CREAL z = a - s; *pr = r - z; *ps = a; } break;
+    case TW_CPST_M3R: { CREAL Y = r; CREAL T = s + Y; CREAL q = T - s; q = Y - q; *pr = q; *ps = T; } break;
+    case TW_CPST_6R: {
+      CREAL a = s + r; CREAL pa = a - r; CREAL pb = a - pa; CREAL da = s - pa; CREAL db = r - pb;
+      *ps = a; *pr = da + db;
+    } break;
+    case TW_CPST_3C: { CREAL Y = r + *pc; CREAL a = s + Y; CREAL z = a - s; *pc = Y - z; *ps = a; *pr = 0; } break;
+    case TW_CPST_M3C: { CREAL Y = r + *pc; CREAL T = s + Y; CREAL q = T - s; q = Y - q; *pc = q; *ps = T; *pr = 0; } break;
+    case TW_CPST_6C: {
+      CREAL Y = r + *pc; CREAL a = s + Y; CREAL pa = a - Y; CREAL pb = a - pa; CREAL da = s - pa; CREAL db = Y - pb;
+      *ps = a; *pc = da + db; *pr = 0;
+    } break;
+  }
+}
+#else
+/* one point of the 16-bit-storage update: FP32 sum t = s + acc, stored
+ * S = RN16(t); K keeps RN16(t - S), K8 the code of t - S */
+static inline void FN(upd_point16)(int cpst, SREAL* ps, SREAL* pr, CREAL acc) {
+  CREAL t = S2C(*ps) + acc;
+  SREAL hi = C2S(t);
+  *ps = hi;
+  if (cpst == TW_CPST_K) *pr = C2S(t - S2C(hi));
+  if (cpst == TW_CPST_K8) *pr = FN(k8_enc)(t - S2C(hi), hi);
+}
+#endif
+
 static void FN(update)(FN(st_t) * S, int f) {
   FN(fld)* F = &S->f[f];
   size_t n = (size_t)F->rows * F->cols;
   int cpst = S->cm->cfg.cpst;
-  (void)cpst;
+  SREAL zero = 0;
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)n; i++) {
 #if REFMODE
-    CREAL s = F->s[i], r = F->acc[i];
-    switch (cpst) {
-      case TW_CPST_0: F->s[i] = s + r; F->r[i] = 0; break;
-      case TW_CPST_3R: { CREAL a = s + r; CREAL z = a - s; F->r[i] = r - z; F->s[i] = a; } break;
-      case TW_CPST_M3R: { CREAL Y = r; CREAL T = s + Y; CREAL q = T - s; q = Y - q; F->r[i] = q; F->s[i] = T; } break;
-      case TW_CPST_6R: {
-        CREAL a = s + r; CREAL pa = a - r; CREAL pb = a - pa; CREAL da = s - pa; CREAL db = r - pb;
-        F->s[i] = a; F->r[i] = da + db;
-      } break;
-      case TW_CPST_3C: { CREAL Y = r + F->c[i]; CREAL a = s + Y; CREAL z = a - s; F->c[i] = Y - z; F->s[i] = a; F->r[i] = 0; } break;
-      case TW_CPST_M3C: { CREAL Y = r + F->c[i]; CREAL T = s + Y; CREAL q = T - s; q = Y - q; F->c[i] = q; F->s[i] = T; F->r[i] = 0; } break;
-      case TW_CPST_6C: {
-        CREAL Y = r + F->c[i]; CREAL a = s + Y; CREAL pa = a - Y; CREAL pb = a - pa; CREAL da = s - pa; CREAL db = Y - pb;
-        F->s[i] = a; F->c[i] = da + db; F->r[i] = 0;
-      } break;
-    }
+    SREAL cdummy = zero;
+    FN(upd_point)(cpst, &F->s[i], &F->r[i], F->c ? &F->c[i] : &cdummy, F->acc[i]);
 #else
-    CREAL t = S2C(F->s[i]) + F->acc[i];
-    SREAL hi = C2S(t);
-    F->s[i] = hi;
-    if (cpst == TW_CPST_K) F->r[i] = C2S(t - S2C(hi));
-    if (cpst == TW_CPST_K8) F->r[i] = FN(k8_enc)(t - S2C(hi), hi);
+    SREAL rdummy = zero;
+    FN(upd_point16)(cpst, &F->s[i], F->r ? &F->r[i] : &rdummy, F->acc[i]);
 #endif
   }
 }
@@ -615,6 +644,33 @@ static int FN(table)(void* p, int which, double* out) {
   }
   return -1;
 }
+
+/* single-point entry points of the update, for the tests of the
+ * error-free transformations and of the residual codes (the same functions
+ * the time step calls) */
+#if REFMODE
+static void FN(update_points)(int cpst, long n, double* s, double* r, double* c) {
+  for (long i = 0; i < n; i++) {
+    SREAL ss = (SREAL)s[i], rr = (SREAL)r[i], cc = (SREAL)c[i];
+    FN(upd_point)(cpst, &ss, &rr, &cc, (CREAL)r[i]);
+    s[i] = (double)ss;
+    r[i] = (double)rr;
+    c[i] = (double)cc;
+  }
+}
+#else
+static void FN(store_points)(int cpst, long n, const double* t, double* hi, double* res) {
+  for (long i = 0; i < n; i++) {
+    SREAL ss = C2S((CREAL)0), rr = C2S((CREAL)0);
+    FN(upd_point16)(cpst, &ss, &rr, (CREAL)t[i]);
+    hi[i] = (double)S2C(ss);
+    res[i] = (double)S2C(rr);
+  }
+}
+static void FN(k8_decode_points)(long n, const double* hi, const double* q, double* out) {
+  for (long i = 0; i < n; i++) out[i] = (double)FN(k8_dec)(C2S((CREAL)q[i]), C2S((CREAL)hi[i]));
+}
+#endif
 
 static const or_vtbl FN(vt) = {FN(create), FN(destroy), FN(load_coef), FN(load_z), FN(set_field),
                                FN(get_field), FN(one_step), FN(table)};

@@ -130,6 +130,66 @@ def test_autograd_wrapper():
                                   traces(s, op.state0, [t.detach().numpy() for t in m], 25))
 
 
+def test_autograd_two_forwards_then_backwards():
+    """Two forwards (different models) before their backwards: each gradient
+    is that of its own trajectory, equal to a direct adjoint run there."""
+    torch = pytest.importorskip("torch")
+    import twave.autograd as TA
+    s = acoustic_solver("FCCC", nt=25)
+    op = TA.AcousticTraces(s, 25)
+    base = [s.get_coef(k) for k in COEFS]
+    ma = [torch.tensor(b, requires_grad=True) for b in base]
+    mb = [torch.tensor(b * f, requires_grad=True) for b, f in zip(base, (1.05, 0.97, 1.02))]
+    rng = np.random.default_rng(5)
+    wa = torch.tensor(rng.standard_normal((s.nrcv, 25)))
+    wb = torch.tensor(rng.standard_normal((s.nrcv, 25)))
+    la = (op(*ma) * wa).sum()
+    lb = (op(*mb) * wb).sum()
+    (la + lb).backward()
+    for m, w in ((ma, wa), (mb, wb)):
+        s.restore(op.state0)
+        for k, t in zip(COEFS, m):
+            s.set_coef(k, t.detach().numpy())
+        s.adjoint_forward(25)
+        ref = s.adjoint_run(w.numpy())
+        for t, r in zip(m, ref):
+            np.testing.assert_array_equal(t.grad.numpy(), r)
+
+
+def test_autograd_density_parametrization():
+    """Gradient with respect to rho on the velocity grids equals the chain
+    rule -(1/rho^2) d/d(1/rho) of the direct adjoint, and matches central
+    differences of the traces in rho."""
+    torch = pytest.importorskip("torch")
+    import twave.autograd as TA
+    s = acoustic_solver("FCCC", nt=25)
+    op = TA.AcousticTracesKappaRho(s, 25)
+    kap = torch.tensor(s.get_coef(T.C_KAPPA), requires_grad=True)
+    rho = [torch.tensor(1.0 / s.get_coef(k), requires_grad=True) for k in (T.C_BINV_VX, T.C_BINV_VY)]
+    w = torch.tensor(np.random.default_rng(6).standard_normal((s.nrcv, 25)))
+    (op(kap, *rho) * w).sum().backward()
+    s.restore(op.state0)
+    for k, t in zip(COEFS, (kap, 1.0 / rho[0], 1.0 / rho[1])):
+        s.set_coef(k, t.detach().numpy())
+    s.adjoint_forward(25)
+    gk, gx, gy = s.adjoint_run(w.numpy())
+    np.testing.assert_array_equal(kap.grad.numpy(), gk)
+    for t, g in zip(rho, (gx, gy)):
+        r = t.detach().numpy()
+        np.testing.assert_allclose(t.grad.numpy(), -g / r ** 2, rtol=1e-14, atol=0)
+    d = [np.random.default_rng(7 + i).standard_normal(t.shape) for i, t in enumerate(rho)]
+    eps = 1e-6
+
+    def loss(sign):
+        r = [torch.tensor(t.detach().numpy() + sign * eps * dd) for t, dd in zip(rho, d)]
+        with torch.no_grad():
+            return float((op(kap.detach(), *r) * w).sum())
+
+    fd = (loss(1) - loss(-1)) / (2 * eps)
+    ad = sum(float((t.grad * torch.tensor(dd)).sum()) for t, dd in zip(rho, d))
+    assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+
+
 class C5Scaled:
     """Scaled-down adjoint configuration: true model = unit background
     velocity plus a circular 10% anomaly, starting model = a 3% slower

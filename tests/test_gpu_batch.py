@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import common as C
+import oracle as O
 import twave as T
 
 pytestmark = pytest.mark.gpu
@@ -81,6 +82,46 @@ def test_batch_equals_solo_elastic():
     assert_same(batched, solo)
 
 
+def oracle_twin(physics, precision, cpst, mx, my, bc, seed, nt=200, coef16=0):
+    """The CPU oracle set up like problem(): same configuration, the
+    coefficients the solver decoded (8-bit codes: FP32 coefficients set to the
+    decoded values), the same seeded fields and source."""
+    kw = C.cfg_kwargs(physics=physics, mx=mx, my=my, bc=bc, precision=precision, cpst=cpst, nt=nt,
+                      stencil_form=1 if physics == "acoustic" else 0, coef16=0 if coef16 == 2 else coef16)
+    o = O.Oracle(O.make_config(**kw))
+    s = problem(physics, precision, cpst, mx, my, bc, seed, nt=nt, coef16=coef16)
+    ks = [T.C_BINV_VX, T.C_BINV_VY, T.C_KAPPA] if physics == "acoustic" else \
+         [T.C_BINV_VX, T.C_BINV_VY, T.C_MU, T.C_LAMBDA, T.C_TWOMU]
+    for k in ks:
+        o.set_coef(k, s.get_coef(k))
+    for f in s.fields:
+        o.set_field(f, s.get_field(f))
+    grid = T.GRID_MM if physics == "acoustic" else T.GRID_NM
+    o.set_source_ricker(grid, mx // 2, my // 3, 25.0, 0.01)
+    s.close()
+    return o
+
+
+@pytest.mark.parametrize("physics,precision,cpst,coef16", [("acoustic", "bf16", "K8", 1), ("acoustic", "bf16", "0", 2),
+                                                           ("elastic", "bf16", "K8", 2), ("acoustic", "fp32", "6R", 0)])
+def test_batch_equals_oracle(physics, precision, cpst, coef16):
+    """Batched launches against the CPU oracle directly (not only against the
+    same problems stepped alone): every field and companion bitwise after a
+    graph-captured step range and a second range."""
+    print("case batch vs oracle", physics, precision, cpst, coef16, flush=True)
+    shapes = SHAPES if physics == "acoustic" else [(40, 48, "FFPP"), (33, 70, "PPPP"), (64, 300, "FCCC")]
+    batched = [problem(physics, precision, cpst, *sh, seed=i, coef16=coef16) for i, sh in enumerate(shapes)]
+    twins = [oracle_twin(physics, precision, cpst, *sh, seed=i, coef16=coef16) for i, sh in enumerate(shapes)]
+    B = T.Batch(batched)
+    B.step(25)
+    B.step(15)
+    for o in twins:
+        o.step(40)
+    for x, o in zip(batched, twins):
+        C.compare_state(x, o, companions=True)
+    B.close()
+
+
 def test_batch_graph_replay():
     batched, _ = pairs("acoustic", "bf16", "K")
     blobs = [s.checkpoint() for s in batched]
@@ -93,6 +134,37 @@ def test_batch_graph_replay():
     for s, ref in zip(batched, first):
         for f in s.fields:
             np.testing.assert_array_equal(s.get_field(f), ref[f])
+
+
+def test_batch_graph_rebuilt_after_new_source():
+    """Reset, set the next shot's source, step again from the same step: the
+    batch must not replay the graph captured with the previous source."""
+    batched, solo = pairs("acoustic", "bf16", "K8")
+    blobs = [s.checkpoint() for s in batched]
+    B = T.Batch(batched)
+    B.step(12)
+    for s, blob in zip(batched, blobs):
+        s.restore(blob)
+        s.set_source_ricker(T.GRID_MM, s.shape(T.P)[0] // 3, s.shape(T.P)[1] // 2, 18.0, 0.02)
+    for s in solo:
+        s.set_source_ricker(T.GRID_MM, s.shape(T.P)[0] // 3, s.shape(T.P)[1] // 2, 18.0, 0.02)
+    B.step(12)
+    for s in solo:
+        s.step(12)
+    assert_same(batched, solo)
+    # and new coefficients between replays
+    for s, blob in zip(batched, blobs):
+        s.restore(blob)
+    for x, y in zip(batched, solo):
+        k = T.C_KAPPA
+        v = 1.1 * x.get_coef(k)
+        x.set_coef(k, v)
+        y.restore(x.checkpoint())
+        y.set_coef(k, v)
+    B.step(12)
+    for s in solo:
+        s.step(12)
+    assert_same(batched, solo)
 
 
 def test_batch_autotune_keeps_state():

@@ -115,6 +115,33 @@ def test_run_case_files_match_oracle(tmp_path, with_models):
         assert np.abs(e).max() > 0
 
 
+def test_write_results_requires_the_shots_receivers(tmp_path):
+    """Results are written only when the solver records exactly the receivers
+    of the shot named (count, fields and indices); otherwise TW_ESTATE, never
+    a write past the records or traces under other receivers' names."""
+    (tmp_path / "InputFile.txt").write_text(INPUT.format(nt=10))
+    inp = FE.read_input_file(str(tmp_path / "InputFile.txt"))
+    cfg = FE.input_to_config(inp)
+    s = T.Solver(cfg)
+    (tmp_path / "a.txt").write_text("Src_1_MN_41_250_20_250\nRcv_1_1_MN_61_250_20_250\n"
+                                    "Src_2_MN_81_250_20_250\nRcv_2_1_MN_61_250_20_250\nRcv_2_2_MN_71_250_30_250\n")
+    entries, n = FE.read_srcrcv_file(str(tmp_path / "a.txt"))
+    FE.setup_shot(s, inp, entries, n, 1)
+    f, x, y = s.get_receivers()
+    assert len(f) == s.nrcv > 0
+    s.step(5)
+    FE.write_results(s, str(tmp_path), entries, n, 1)
+    with pytest.raises(T.TwError) as e:
+        FE.write_results(s, str(tmp_path), entries, n, 2)  # shot 2 has more receivers
+    assert e.value.code == 6  # TW_ESTATE
+    s.set_receivers(list(f), list(x + 1), list(y))  # same count, other points
+    s.step(1)
+    with pytest.raises(T.TwError) as e:
+        FE.write_results(s, str(tmp_path), entries, n, 1)
+    assert e.value.code == 6  # TW_ESTATE
+    s.close()
+
+
 def test_setup_shot_rejects(tmp_path):
     (tmp_path / "InputFile.txt").write_text(INPUT.format(nt=10))
     inp = FE.read_input_file(str(tmp_path / "InputFile.txt"))

@@ -213,7 +213,8 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.mark.parametrize("prec,name", [("fp64", "f64"), ("fp32", "f32")])
-def test_gpu_against_reference_golden(prec, name):
+@pytest.mark.parametrize("cpst", ["0", "3R", "6R"])
+def test_gpu_against_reference_golden(prec, name, cpst):
     """The GPU solver on the reference's own single-core acoustic case
     (singlecore/src/main_cpp/cpu_acoustic.cpp: 600^2 periodic, dx = 0.008,
     dt = 1e-4, Ricker f = 5, t0 = 0.25 on P at (200, 200), receivers at
@@ -221,11 +222,13 @@ def test_gpu_against_reference_golden(prec, name):
     (tests/golden, "% 16.15e"): receiver traces digit for digit over the
     first 20000 steps (the wave arrives after ~17000), energies
     E = E_Vx + E_Vy + (E_P(n) + E_P(n-1))/2 to 1e-12 (the summation order
-    differs)."""
-    g = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s.npz" % name))
+    differs).  cpst 3R / 6R against the reference run with -cpst 3 / -cpst 6
+    (Fast2Sum / 2Sum, singlecore/src/cpp/namespaces.hpp:633-668)."""
+    suffix = "" if cpst == "0" else "_cpst" + cpst[0]
+    g = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s%s.npz" % (name, suffix)))
     nt = 20000
     cfg = T.config(physics=T.ACOUSTIC, mx=600, my=600, dx=(1.0 / (5.0 * 2.5)) / 10, num_dx=0, den_dx=0, dt=1e-4,
-                   bc="PPPP", precision=prec, cpst="0", stencil_form=1, energy_form=1, nt_max=nt)
+                   bc="PPPP", precision=prec, cpst=cpst, stencil_form=1, energy_form=1, nt_max=nt)
     s = T.Solver(cfg)
     s.set_source_ricker(T.GRID_MM, 200, 200, 5.0, 0.25)
     s.set_receivers([T.VX, T.VY, T.P], [400, 400, 400], [400, 400, 400])

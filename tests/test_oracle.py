@@ -42,8 +42,8 @@ def fmt(v):
     return np.array([float("% 16.15e" % x) for x in v])
 
 
-def reference_acoustic_run(prec, nt):
-    cfg = O.acoustic_reference_config(precision=prec, nt_max=nt)
+def reference_acoustic_run(prec, nt, cpst="0"):
+    cfg = O.acoustic_reference_config(precision=prec, nt_max=nt, cpst=cpst)
     o = O.Oracle(cfg)
     # source on P at (200, 200), f = 5, t0 = 0.25; receivers at (400, 400)
     # (singlecore/src/cpp/namespaces.hpp class_mesh::initialize; cpu_acoustic.cpp:88-104)
@@ -64,6 +64,38 @@ def test_golden_energy_head(prec, name):
     n = 1000
     E, _ = reference_acoustic_run(prec, n)
     np.testing.assert_array_equal(fmt(E), g["E_head"][:n])
+
+
+@pytest.mark.parametrize("prec,name", [(O.FP64, "f64"), (O.FP32, "f32")])
+@pytest.mark.parametrize("c", ["3", "6"])
+def test_golden_compensated_head(prec, name, c):
+    """The reference's compensated updates (-cpst 3: Fast2Sum, -cpst 6: 2Sum,
+    singlecore/src/cpp/namespaces.hpp:633-668, run through oracle/ref_cli.cpp):
+    the oracle's 3R / 6R reproduce the first 1000 steps of each energy history
+    digit for digit, and each fixture tells the two variants apart (in FP32 the
+    histories differ from step ~95 on), so a 3 <-> 6 swap fails here."""
+    g = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s_cpst%s.npz" % (name, c)))
+    n = 1000
+    E, _ = reference_acoustic_run(prec, n, c + "R")
+    np.testing.assert_array_equal(fmt(E), g["E_head"][:n])
+    other = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s_cpst%s.npz" % (name, "6" if c == "3" else "3")))
+    plain = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s.npz" % name))
+    assert (g["E_head"][:n] != plain["E_head"][:n]).any()
+    if prec == O.FP32:
+        assert (g["E_head"][:n] != other["E_head"][:n]).any()
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not os.environ.get("TW_RUN_SLOW"), reason="full 20000-step runs (~5 min); set TW_RUN_SLOW=1")
+@pytest.mark.parametrize("prec,name", [(O.FP64, "f64"), (O.FP32, "f32")])
+@pytest.mark.parametrize("c", ["3", "6"])
+def test_golden_compensated_full(prec, name, c):
+    g = np.load(os.path.join(GOLDEN, "acoustic_singlecore_%s_cpst%s.npz" % (name, c)))
+    E, rec = reference_acoustic_run(prec, 20000, c + "R")
+    np.testing.assert_array_equal(fmt(E[::50]), g["E_every"])
+    it = g["trace_it"]
+    for k, f in enumerate(("Vx", "Vy", "P")):
+        np.testing.assert_array_equal(fmt(rec[k][it]), g[f])
 
 
 @pytest.mark.slow

@@ -296,6 +296,9 @@ struct tw_solver {
   std::vector<tw::Receiver> rcv_v, rcv_s;
   int nrcv = 0;
   int it = 0;
+  // bumped by every change that a batch's captured tables depend on (source,
+  // receivers, coefficients): a batch rebuilds its graph when it differs
+  uint64_t gen = 0;
   bool ep_valid = false;
   bool need_uhat = false;
   // discrete adjoint: checkpoints of the last checkpointed forward run
@@ -443,6 +446,7 @@ int upload_z(tw_solver* s) {
 
 int after_coef_change(tw_solver* s) {
   s->ep_valid = false;
+  s->gen++;
   return upload_z(s);
 }
 
@@ -762,11 +766,13 @@ int tw_set_coef(tw_solver* s, int32_t k, const double* v) {
   if (!s || !v) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
   if (!coef_in(s->cfg.physics, k)) return fail(&s->err, TW_EINVAL, "coefficient not used by this physics");
   (void)hipSetDevice(s->cfg.device);
-  for (size_t i = 0; i < s->hcoef[k].size(); i++) {
+  const size_t n = s->hcoef[k].size();
+  // validate the whole array first: a refused call leaves the coefficients as they were
+  for (size_t i = 0; i < n; i++) {
     if (!std::isfinite(v[i])) return fail(&s->err, TW_EINVAL, "coefficient must be finite");
     if (!(v[i] > 0) && k != TW_C_LAMBDA) return fail(&s->err, TW_EINVAL, "coefficient must be positive");
-    s->hcoef[k][i] = round_coef(&s->cfg, v[i]);
   }
+  for (size_t i = 0; i < n; i++) s->hcoef[k][i] = round_coef(&s->cfg, v[i]);
   int e = upload_coef(s, k);
   if (e) return e;
   return after_coef_change(s);
@@ -804,6 +810,7 @@ int tw_set_source_ricker(tw_solver* s, int32_t grid, int32_t ix, int32_t iy, dou
   s->src_grid = grid;
   s->src_ix = ix;
   s->src_iy = iy;
+  s->gen++;
   return TW_OK;
 }
 
@@ -815,6 +822,7 @@ int tw_set_source_wavelet(tw_solver* s, int32_t grid, int32_t ix, int32_t iy, co
   s->src_grid = grid;
   s->src_ix = ix;
   s->src_iy = iy;
+  s->gen++;
   return TW_OK;
 }
 
@@ -822,6 +830,7 @@ int tw_clear_source(tw_solver* s) {
   if (!s) return fail(nullptr, TW_EINVAL, "null handle");
   s->src_grid = -1;
   s->src_inc.clear();
+  s->gen++;
   return TW_OK;
 }
 
@@ -854,6 +863,21 @@ int tw_set_receivers(tw_solver* s, int32_t n, const int32_t* field, const int32_
   s->plan.nrcv_v = (int32_t)v.size();
   s->plan.nrcv_s = (int32_t)st.size();
   s->nrcv = n;
+  s->gen++;
+  return TW_OK;
+}
+
+int tw_get_receivers(tw_solver* s, int32_t cap, int32_t* n, int32_t* field, int32_t* ix, int32_t* iy) {
+  if (!s || !n || cap < 0) return fail(s ? &s->err : nullptr, TW_EINVAL, "bad argument");
+  *n = s->nrcv;
+  if (!field && !ix && !iy) return TW_OK;
+  if (cap < s->nrcv) return fail(&s->err, TW_ERANGE, "more receivers than cap");
+  for (const auto* list : {&s->rcv_v, &s->rcv_s})
+    for (const tw::Receiver& r : *list) {
+      if (field) field[r.row] = r.field;
+      if (ix) ix[r.row] = r.ix;
+      if (iy) iy[r.row] = r.iy;
+    }
   return TW_OK;
 }
 
@@ -1535,6 +1559,7 @@ struct tw_batch {
   hipGraphExec_t exec = nullptr;
   int g_nsteps = -1, g_rows = -1;
   std::vector<int> g_it0;
+  std::vector<uint64_t> g_gen;  // each solver's generation when the graph was built
   void* items = nullptr;
   size_t items_cap = 0;
   // per step and half: item offsets and launch sizes
@@ -1644,14 +1669,18 @@ int batch_build(tw_batch* b, int nsteps) {
   b->g_nsteps = nsteps;
   b->g_rows = b->rows;
   b->g_it0.clear();
-  for (tw_solver* s : b->sv) b->g_it0.push_back(s->it);
+  b->g_gen.clear();
+  for (tw_solver* s : b->sv) {
+    b->g_it0.push_back(s->it);
+    b->g_gen.push_back(s->gen);
+  }
   return TW_OK;
 }
 
 bool batch_cached(const tw_batch* b, int nsteps) {
   if (!b->exec || b->g_nsteps != nsteps || b->g_rows != b->rows) return false;
   for (size_t k = 0; k < b->sv.size(); k++)
-    if (b->g_it0[k] != b->sv[k]->it) return false;
+    if (b->g_it0[k] != b->sv[k]->it || b->g_gen[k] != b->sv[k]->gen) return false;
   return true;
 }
 

@@ -214,7 +214,8 @@ struct BatchBand {
   int32_t nb;
 };
 
-// launches (tw_kernels.hip)
+// launches (tw_kernels.hip, one build per storage type; the entry points that
+// depend on it are routed by tw_dispatch.cpp)
 void grid_dims(const Plan& p, dim3* grid, dim3* block);
 int energy_parts(const Plan& p);  // partial sums written by launch_stress_energy
 int half_parts(const Plan& p);    // partial sums written per half step
@@ -231,13 +232,13 @@ hipError_t launch_uhat(const Plan& p, hipStream_t s);
 hipError_t launch_stress_energy(const Plan& p, hipStream_t s);
 hipError_t launch_finalize(const Plan& p, const Pending& pend, hipStream_t s);
 hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s);
-// one backward step of the acoustic adjoint for forward step `it`: inject the
-// receiver sources of step it, then apply the transposed step using the saved
-// states after (cur) and before (prev) step it, accumulating the gradients
 hipError_t launch_batch_half(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
                              const BatchBand* band, int nband, int band_blocks, hipStream_t s);
 // core and band share of one problem (d, sp and b0 are left to the caller)
 void batch_geometry(const Plan& p, BatchCore* core, int* core_blocks, BatchBand* band, int* band_blocks);
+// one backward step of the acoustic adjoint for forward step `it`: inject the
+// receiver sources of step it, then apply the transposed step using the saved
+// states after (cur) and before (prev) step it, accumulating the gradients
 hipError_t launch_adjoint_step(const Plan& p, const AdjPlan& a, const StateView& cur, const StateView& prev, int it,
                                int it_rel, hipStream_t s);
 

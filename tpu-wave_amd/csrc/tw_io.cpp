@@ -413,6 +413,14 @@ int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* en
   if (e) return e;
   Shot sh;
   if ((e = build_shot(&cfg, entries, n, src_index, &sh))) return e;
+  // the solver must record exactly this shot's receivers, in the order the
+  // files are named after (tw_setup_shot registers them so)
+  int32_t nreg = 0;
+  if ((e = tw_get_receivers(s, 0, &nreg, nullptr, nullptr, nullptr))) return e;
+  if (nreg != (int32_t)sh.field.size()) return err(TW_ESTATE, "the solver records other receivers than this shot's");
+  std::vector<int32_t> rf(nreg), rx(nreg), ry(nreg);
+  if (nreg > 0 && (e = tw_get_receivers(s, nreg, &nreg, rf.data(), rx.data(), ry.data()))) return e;
+  if (rf != sh.field || rx != sh.fx || ry != sh.fy) return err(TW_ESTATE, "the solver records other receivers than this shot's");
   int32_t it = 0;
   if ((e = tw_current_step(s, &it))) return e;
   const int nt = it < cfg.nt_max ? it : cfg.nt_max;

@@ -27,6 +27,41 @@
 
 #include "tw_internal.h"
 
+// This file is compiled once per field storage type, TW_PREC = 0..3 (FP64,
+// FP32, BF16, FP16), so the four sets of template instantiations build in
+// parallel.  Each build defines the precision-dispatched entry points with the
+// suffix _p<TW_PREC> (TW_PUB); tw_dispatch.cpp routes the names declared in
+// tw_internal.h to them.  Geometry helpers without a storage type are defined
+// by the TW_PREC == 0 build only.
+#ifndef TW_PREC
+#error "compile with -DTW_PREC=<0..3>"
+#endif
+#define TW_PNAME_(n, p) n##_p##p
+#define TW_PNAME(n, p) TW_PNAME_(n, p)
+#define TW_PUB(n) TW_PNAME(n, TW_PREC)
+
+// return F<ST, KT, CT>(args) for this build's storage type and the plan's
+// coefficient storage (FP32, the 16-bit type, or 8-bit codes)
+#if TW_PREC == 0
+#define TW_DISPATCH(p, F, ...) return F<double, double, double>(__VA_ARGS__)
+#elif TW_PREC == 1
+#define TW_DISPATCH(p, F, ...) return F<float, float, float>(__VA_ARGS__)
+#elif TW_PREC == 2
+#define TW_DISPATCH(p, F, ...)                                     \
+  do {                                                             \
+    if ((p).coef16 == 2) return F<bf16, c8, float>(__VA_ARGS__);   \
+    if ((p).coef16) return F<bf16, bf16, float>(__VA_ARGS__);      \
+    return F<bf16, float, float>(__VA_ARGS__);                     \
+  } while (0)
+#else
+#define TW_DISPATCH(p, F, ...)                                             \
+  do {                                                                     \
+    if ((p).coef16 == 2) return F<_Float16, c8, float>(__VA_ARGS__);       \
+    if ((p).coef16) return F<_Float16, _Float16, float>(__VA_ARGS__);      \
+    return F<_Float16, float, float>(__VA_ARGS__);                         \
+  } while (0)
+#endif
+
 namespace tw {
 
 // ------------------------------------------------------------ storage types
@@ -2246,6 +2281,7 @@ static bool use_stream(const Plan& p) {
   return p.kver != 1 && !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C);
 }
 
+#if TW_PREC == 0
 void grid_dims(const Plan& p, dim3* grid, dim3* block) {
   if (!use_stream(p)) {
     *block = dim3(64, 4, 1);
@@ -2270,16 +2306,20 @@ int half_parts(const Plan& p) {
   return (int)(g.x * g.y) + R.cum[R.n];
 }
 
+#endif
+
 static void grid_dims_v1(const Plan& p, dim3* grid, dim3* block) {
   *block = dim3(64, 4, 1);
   *grid = dim3((unsigned)((p.my + 1 + 63) / 64), (unsigned)((p.mx + 1 + 3) / 4), 1);
 }
 
+#if TW_PREC == 0
 int energy_parts(const Plan& p) {
   dim3 g, b;
   grid_dims_v1(p, &g, &b);
   return (int)(g.x * g.y);
 }
+#endif
 
 template <class ST, class KT, class CT, int PH, int CP>
 static void launch_core_t(int half, const Stat<ST, KT, CT>* sp, const Dyn& d, const Core& C, dim3 g, dim3 b,
@@ -2387,20 +2427,7 @@ static size_t stat_fill_t(const Plan& p, void* out, int pp) {
   return sizeof(Stat<ST, KT, CT>);
 }
 
-size_t stat_fill(const Plan& p, void* out, int pp) {
-  switch (p.precision) {
-    case TW_FP64: return stat_fill_t<double, double, double>(p, out, pp);
-    case TW_FP32: return stat_fill_t<float, float, float>(p, out, pp);
-    case TW_BF16:
-      if (p.coef16 == 2) return stat_fill_t<bf16, c8, float>(p, out, pp);
-      if (p.coef16) return stat_fill_t<bf16, bf16, float>(p, out, pp);
-      return stat_fill_t<bf16, float, float>(p, out, pp);
-    default:
-      if (p.coef16 == 2) return stat_fill_t<_Float16, c8, float>(p, out, pp);
-      if (p.coef16) return stat_fill_t<_Float16, _Float16, float>(p, out, pp);
-      return stat_fill_t<_Float16, float, float>(p, out, pp);
-  }
-}
+size_t TW_PUB(stat_fill)(const Plan& p, void* out, int pp) { TW_DISPATCH(p, stat_fill_t, p, out, pp); }
 
 
 template <class ST, class KT, class CT>
@@ -2421,40 +2448,19 @@ static hipError_t run_misc(const Plan& p, int what, const Pending* pend, hipStre
   return hipGetLastError();
 }
 
-hipError_t launch_half(const Plan& p, int half, const StepCtl& c, hipStream_t s) {
-  switch (p.precision) {
-    case TW_FP64: return run_half<double, double, double>(p, half, c, s);
-    case TW_FP32: return run_half<float, float, float>(p, half, c, s);
-    case TW_BF16:
-      if (p.coef16 == 2) return run_half<bf16, c8, float>(p, half, c, s);
-      if (p.coef16) return run_half<bf16, bf16, float>(p, half, c, s);
-      return run_half<bf16, float, float>(p, half, c, s);
-    default:
-      if (p.coef16 == 2) return run_half<_Float16, c8, float>(p, half, c, s);
-      if (p.coef16) return run_half<_Float16, _Float16, float>(p, half, c, s);
-      return run_half<_Float16, float, float>(p, half, c, s);
-  }
+hipError_t TW_PUB(launch_half)(const Plan& p, int half, const StepCtl& c, hipStream_t s) {
+  TW_DISPATCH(p, run_half, p, half, c, s);
 }
 
 static hipError_t misc(const Plan& p, int what, const Pending* pend, hipStream_t s) {
-  switch (p.precision) {
-    case TW_FP64: return run_misc<double, double, double>(p, what, pend, s);
-    case TW_FP32: return run_misc<float, float, float>(p, what, pend, s);
-    case TW_BF16:
-      if (p.coef16 == 2) return run_misc<bf16, c8, float>(p, what, pend, s);
-      if (p.coef16) return run_misc<bf16, bf16, float>(p, what, pend, s);
-      return run_misc<bf16, float, float>(p, what, pend, s);
-    default:
-      if (p.coef16 == 2) return run_misc<_Float16, c8, float>(p, what, pend, s);
-      if (p.coef16) return run_misc<_Float16, _Float16, float>(p, what, pend, s);
-      return run_misc<_Float16, float, float>(p, what, pend, s);
-  }
+  TW_DISPATCH(p, run_misc, p, what, pend, s);
 }
 
-hipError_t launch_stress_energy(const Plan& p, hipStream_t s) { return misc(p, 0, nullptr, s); }
-hipError_t launch_uhat(const Plan& p, hipStream_t s) { return misc(p, 1, nullptr, s); }
-hipError_t launch_finalize(const Plan& p, const Pending& pend, hipStream_t s) { return misc(p, 2, &pend, s); }
+hipError_t TW_PUB(launch_stress_energy)(const Plan& p, hipStream_t s) { return misc(p, 0, nullptr, s); }
+hipError_t TW_PUB(launch_uhat)(const Plan& p, hipStream_t s) { return misc(p, 1, nullptr, s); }
+hipError_t TW_PUB(launch_finalize)(const Plan& p, const Pending& pend, hipStream_t s) { return misc(p, 2, &pend, s); }
 
+#if TW_PREC == 0
 hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s) {
   dim3 b(64, 4);
   for (int f = 0; f < NF; f++) {
@@ -2482,6 +2488,7 @@ hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s) {
   }
   return hipGetLastError();
 }
+#endif
 
 // ------------------------------------------------------------ fused step
 // One pass of the elastic leapfrog step over the interior, both axes SBP: a
@@ -2790,16 +2797,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(
 }
 
 // ---- fused step launches
+#if TW_PREC == 0
 bool fused_ok(const Plan& p) {
   return p.physics == TW_ELASTIC && !p.perx && !p.pery && p.energy_form == 0 && p.mx >= 12 && p.my >= 12 &&
          p.coef16 != 2 &&
          !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C || p.cpst == TW_CPST_K8);
 }
 
+#endif
+
 static void fused_dims(const Plan& p, const Core& C, dim3* g) {
   *g = dim3((unsigned)((C.chi - C.clo + FSTRIDE - 1) / FSTRIDE), (unsigned)((C.rhi - C.rlo + p.txc - 1) / p.txc), 1);
 }
 
+#if TW_PREC == 0
 int fused_parts(const Plan& p) {
   Core C;
   Rects R;
@@ -2808,6 +2819,7 @@ int fused_parts(const Plan& p) {
   fused_dims(p, C, &g);
   return (int)(g.x * g.y) + R.cum[R.n];
 }
+#endif
 
 template <class ST, class KT, class CT>
 static hipError_t run_fused(const Plan& p, int what, const StepCtl& c, hipStream_t s) {
@@ -2862,17 +2874,20 @@ static hipError_t run_fused(const Plan& p, int what, const StepCtl& c, hipStream
   return hipGetLastError();
 }
 
-hipError_t launch_fused(const Plan& p, int what, const StepCtl& c, hipStream_t s) {
-  switch (p.precision) {
-    case TW_FP64: return run_fused<double, double, double>(p, what, c, s);
-    case TW_FP32: return run_fused<float, float, float>(p, what, c, s);
-    case TW_BF16:
-      if (p.coef16) return run_fused<bf16, bf16, float>(p, what, c, s);
-      return run_fused<bf16, float, float>(p, what, c, s);
-    default:
-      if (p.coef16) return run_fused<_Float16, _Float16, float>(p, what, c, s);
-      return run_fused<_Float16, float, float>(p, what, c, s);
-  }
+hipError_t TW_PUB(launch_fused)(const Plan& p, int what, const StepCtl& c, hipStream_t s) {
+  // the fused step takes no 8-bit coefficients (fused_ok)
+  if (p.coef16 == 2) return hipErrorInvalidValue;
+#if TW_PREC == 0
+  return run_fused<double, double, double>(p, what, c, s);
+#elif TW_PREC == 1
+  return run_fused<float, float, float>(p, what, c, s);
+#elif TW_PREC == 2
+  if (p.coef16) return run_fused<bf16, bf16, float>(p, what, c, s);
+  return run_fused<bf16, float, float>(p, what, c, s);
+#else
+  if (p.coef16) return run_fused<_Float16, _Float16, float>(p, what, c, s);
+  return run_fused<_Float16, float, float>(p, what, c, s);
+#endif
 }
 
 // ------------------------------------------------------------ batched shots
@@ -2984,24 +2999,12 @@ static hipError_t run_batch_half(const Plan& p, int half, const BatchCore* core,
   return hipGetLastError();
 }
 
-hipError_t launch_batch_half(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
-                             const BatchBand* band, int nband, int band_blocks, hipStream_t s) {
-  switch (p.precision) {
-    case TW_FP64: return run_batch_half<double, double, double>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-    case TW_FP32: return run_batch_half<float, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-    case TW_BF16:
-      if (p.coef16 == 2) return run_batch_half<bf16, c8, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-      if (p.coef16) return run_batch_half<bf16, bf16, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-      return run_batch_half<bf16, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-    default:
-      if (p.coef16 == 2)
-        return run_batch_half<_Float16, c8, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-      if (p.coef16)
-        return run_batch_half<_Float16, _Float16, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-      return run_batch_half<_Float16, float, float>(p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
-  }
+hipError_t TW_PUB(launch_batch_half)(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
+                                     const BatchBand* band, int nband, int band_blocks, hipStream_t s) {
+  TW_DISPATCH(p, run_batch_half, p, half, core, ncore, core_blocks, band, nband, band_blocks, s);
 }
 
+#if TW_PREC == 0
 // batch items of one problem for one half step (host): core geometry and
 // band rectangles as the single-problem launch uses them
 void batch_geometry(const Plan& p, BatchCore* core, int* core_blocks, BatchBand* band, int* band_blocks) {
@@ -3019,6 +3022,7 @@ void batch_geometry(const Plan& p, BatchCore* core, int* core_blocks, BatchBand*
   band->nb = R.cum[R.n];
   *band_blocks = band->nb;
 }
+#endif
 
 // ------------------------------------------------------------ adjoint
 // Discrete adjoint of one acoustic leapfrog step (no compensation), the
@@ -3236,13 +3240,15 @@ static hipError_t run_adjoint_step(const Plan& p, const AdjPlan& a, const StateV
   return hipGetLastError();
 }
 
-hipError_t launch_adjoint_step(const Plan& p, const AdjPlan& a, const StateView& cur, const StateView& prev, int it,
-                               int it_rel, hipStream_t s) {
-  switch (p.precision) {
-    case TW_FP64: return run_adjoint_step<double, double, double>(p, a, cur, prev, it, it_rel, s);
-    case TW_FP32: return run_adjoint_step<float, float, float>(p, a, cur, prev, it, it_rel, s);
-    default: return hipErrorInvalidValue;
-  }
+hipError_t TW_PUB(launch_adjoint_step)(const Plan& p, const AdjPlan& a, const StateView& cur, const StateView& prev,
+                                       int it, int it_rel, hipStream_t s) {
+  // the adjoint runs FP64 and FP32 storage only
+#if TW_PREC <= 1
+  TW_DISPATCH(p, run_adjoint_step, p, a, cur, prev, it, it_rel, s);
+#else
+  (void)p, (void)a, (void)cur, (void)prev, (void)it, (void)it_rel, (void)s;
+  return hipErrorInvalidValue;
+#endif
 }
 
 }  // namespace tw

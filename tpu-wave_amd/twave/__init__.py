@@ -120,6 +120,7 @@ _SIGS = [
                                              ctypes.c_int32]),
     ("tw_clear_source", ctypes.c_int, [_vp]),
     ("tw_set_receivers", ctypes.c_int, [_vp, ctypes.c_int32, _ip, _ip, _ip]),
+    ("tw_get_receivers", ctypes.c_int, [_vp, ctypes.c_int32, _ip, _ip, _ip, _ip]),
     ("tw_step", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("tw_synchronize", ctypes.c_int, [_vp]),
     ("tw_current_step", ctypes.c_int, [_vp, _ip]),
@@ -261,6 +262,7 @@ class Solver:
             raise TwError(e, lib().tw_global_error().decode())
         self.h = h
         self.nrcv = 0
+        self._adj_steps = 0
         self.fields = [VX, VY, SXX, SYY, SXY] if cfg.physics == ELASTIC else [VX, VY, P]
 
     def close(self):
@@ -283,11 +285,16 @@ class Solver:
 
     def set_model(self, rho, vp, vs, num_dxm, den_dxm):
         rho, vp, vs = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, vp, vs))
+        if rho.ndim != 2 or vp.shape != rho.shape or (self.cfg.physics == ELASTIC and vs.shape != rho.shape):
+            raise ValueError("rho, vp (and vs) must be 2-D node models of one shape")
         self._chk(lib().tw_set_model(self.h, _d(rho), _d(vp), _d(vs), rho.shape[0] - 1, rho.shape[1] - 1,
                                      num_dxm, den_dxm))
 
     def set_coef(self, k, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
+        want = self.get_coef(k).shape
+        if v.shape != want:
+            raise ValueError("coefficient shape %s, expected %s" % (v.shape, want))
         self._chk(lib().tw_set_coef(self.h, k, _d(v)))
 
     def get_coef(self, k):
@@ -314,6 +321,16 @@ class Solver:
         b = np.ascontiguousarray(iys, dtype=np.int32)
         self._chk(lib().tw_set_receivers(self.h, len(f), _i(f), _i(a), _i(b)))
         self.nrcv = len(f)
+
+    def get_receivers(self):
+        """Registered receivers in record order: (fields, ix, iy)."""
+        n = ctypes.c_int32()
+        self._chk(lib().tw_get_receivers(self.h, 0, ctypes.byref(n), None, None, None))
+        out = [np.zeros(n.value, dtype=np.int32) for _ in range(3)]
+        if n.value:
+            self._chk(lib().tw_get_receivers(self.h, n.value, ctypes.byref(n), *(_i(a) for a in out)))
+        self.nrcv = n.value
+        return tuple(out)
 
     def step(self, n):
         self._chk(lib().tw_step(self.h, n))
@@ -378,12 +395,15 @@ class Solver:
         """Advance nsteps keeping sqrt(N) checkpoints for adjoint_run
         (acoustic, FP64/FP32 storage, cpst 0)."""
         self._chk(lib().tw_adjoint_forward(self.h, int(nsteps), int(interval)))
+        self._adj_steps = int(nsteps)
 
     def adjoint_run(self, rcv_adjoint):
         """dL/d(coefficients) for rcv_adjoint[n_rcv][nsteps] = dL/d(trace) of
         the last adjoint_forward run.  Returns (d/d kappa on MM, d/d(1/rho) on
         NM, d/d(1/rho) on MN)."""
         w = np.ascontiguousarray(rcv_adjoint, dtype=np.float64)
+        if self._adj_steps and (w.ndim != 2 or w.shape[0] != self.nrcv or w.shape[1] != self._adj_steps):
+            raise ValueError("rcv_adjoint shape %s, expected (%d, %d)" % (w.shape, self.nrcv, self._adj_steps))
         gk = np.zeros(self.get_coef(C_KAPPA).shape)
         gx = np.zeros(self.get_coef(C_BINV_VX).shape)
         gy = np.zeros(self.get_coef(C_BINV_VY).shape)
@@ -398,6 +418,9 @@ class Solver:
         """(per-record sum of (trace - data)^2, total) over the first
         data.shape[1] recorded steps (tw_data_misfit)."""
         d = np.ascontiguousarray(data, dtype=np.float64)
+        if d.ndim != 2 or d.shape[0] != self.nrcv or d.shape[1] > self.current_step() or \
+                d.shape[1] > self.cfg.nt_max:
+            raise ValueError("data shape %s: expected (%d, nt) with nt <= the recorded steps" % (d.shape, self.nrcv))
         per = np.zeros(self.nrcv)
         tot = ctypes.c_double()
         self._chk(lib().tw_data_misfit(self.h, _d(d), d.shape[1], _d(per), ctypes.byref(tot)))

@@ -68,6 +68,7 @@ def result_folder(inp, cfg, root):
 
 def setup_shot(solver, inp, entries, n, src_index):
     solver._chk(lib().tw_setup_shot(solver.h, ctypes.byref(inp), entries, n, src_index))
+    solver.get_receivers()  # keeps solver.nrcv in step with the registration
 
 
 def write_results(solver, folder, entries, n, src_index):
