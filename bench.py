@@ -127,6 +127,8 @@ def copy_calibration(torch, nbytes, reps):
 
 
 def make_solver(T, W, m, precision, cpst, coef16, nt, model_from=None):
+    """The elastic workload on a new solver; model_from: take that solver's
+    (quantized) coefficients instead of the exact interpolation."""
     cfg = W.elastic_config(m=m, precision=precision, cpst=cpst, coef16=coef16, nt=nt)
     s = T.Solver(cfg)
     W.setup_elastic(s, m, 0.005)
@@ -146,9 +148,10 @@ def make_solver(T, W, m, precision, cpst, coef16, nt, model_from=None):
 
 
 def timed_steps(torch, s, steps, warmup, profile=False):
-    """Device time (s) of `steps` steps on the solver's stream and the wall
-    time; with profile, also the summed durations (ms) of the velocity and
-    stress core launches inside the timed region (HIP events on that stream)."""
+    """Device time (s) of `steps` steps on the solver's stream (after `warmup`
+    untimed steps) and the wall time; with profile, also the summed durations
+    (ms) of the velocity and stress core launches inside the timed region (HIP
+    events on that stream)."""
     stream = torch.cuda.Stream()
     s.set_stream(stream.cuda_stream)
     if warmup:
@@ -170,13 +173,24 @@ def timed_steps(torch, s, steps, warmup, profile=False):
     return e0.elapsed_time(e1) / 1e3, wall, prof
 
 
+def cpu_threads():
+    """Threads a CPU baseline may use: the CPUs this process may run on,
+    bounded by OMP_NUM_THREADS when the host sets it (the share of the host's
+    cores that comes with the GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def cpu_baseline(m, seconds_target=15.0):
     """Oracle (FP32, all host threads) on the same workload, bounded number of steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import twave as T
     import twave.workloads as W
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     cap = 64
     cfg = W.elastic_config(m=m, precision=T.FP32, nt=cap)
@@ -198,56 +212,86 @@ def cpu_baseline(m, seconds_target=15.0):
                       "%d steps (%.1f s) incl. per-step FP64 energy" % (m, m, n, t)}
 
 
-def reference_cpu_sample(timeout_s=150.0):
-    """The reference's own single-core acoustic solver, built from its sources
-    by oracle/ref_build.sh (oracle/_ref/singlecore_acoustic_f32: -O3, NDEBUG,
-    no FMA contraction), run on its hard-wired 600x600 periodic FP32 case until
-    it reports the time of its first 1000 steps; None if it was not built."""
-    import pty
-    import select
+def _reference_loop_seconds(cmd, timeout_s):
+    """Run a reference acoustic binary (it writes its results to ../result)
+    in a scratch folder and return the "loop time" it prints, or None."""
     import subprocess
     import tempfile
-    exe = os.path.join(ROOT, "oracle", "_ref", "singlecore_acoustic_f32")
-    if not os.access(exe, os.X_OK):
-        return None
     with tempfile.TemporaryDirectory() as d:
-        work = os.path.join(d, "a")  # the binary writes its results to ../result
+        work = os.path.join(d, "a")
         os.makedirs(work)
-        # a pseudo-terminal keeps the child's stdio line-buffered
-        master, slave = pty.openpty()
-        p = subprocess.Popen([exe], cwd=work, stdout=slave, stderr=slave, stdin=subprocess.DEVNULL)
-        os.close(slave)
-        t0, secs, buf = time.perf_counter(), None, ""
         try:
-            while secs is None and time.perf_counter() - t0 < timeout_s:
-                r, _, _ = select.select([master], [], [], 1.0)
-                if not r:
-                    if p.poll() is not None:
-                        break
-                    continue
-                try:
-                    chunk = os.read(master, 4096).decode(errors="replace")
-                except OSError:
-                    break
-                buf += chunk
-                lines = buf.split("\n")
-                buf = lines[-1]
-                for line in lines[:-1]:
-                    # printed every 1000 steps; the first one (step 0) times nothing
-                    if "time for 1000 time steps" in line:
-                        v = float(line.split(":")[-1])
-                        if v > 0:
-                            secs = v
-                            break
-        finally:
-            p.kill()
-            p.wait()
-            os.close(master)
-    if secs is None:
-        return None
-    return {"value": 600 * 600 * 1000 / secs, "unit": "grid-point updates/s", "cores": 1, "kind": "reference",
-            "sample": "reference singlecore acoustic solver (singlecore/src/main_cpp/cpu_acoustic.cpp, built by "
-                      "oracle/ref_build.sh), its 600x600 periodic FP32 case, first 1000 steps (%.1f s)" % secs}
+            r = subprocess.run(cmd, cwd=work, capture_output=True, text=True, timeout=timeout_s,
+                               stdin=subprocess.DEVNULL)
+        except (subprocess.TimeoutExpired, OSError):
+            return None
+        if r.returncode != 0:
+            return None
+        for line in r.stdout.splitlines():
+            if line.strip().startswith("loop time"):
+                return float(line.split(":")[-1])
+    return None
+
+
+def _grid_dims(n):
+    a = int(math.sqrt(n))
+    while n % a:
+        a -= 1
+    return n // a, a
+
+
+def reference_cpu_samples(threads, timeout_s=120.0):
+    """The reference's own acoustic CPU solvers, built from their sources by
+    oracle/ref_build.sh, on their 600x600 periodic FP32 case
+    (singlecore/src/main_cpp/cpu_acoustic.cpp, mpi/src/main_cpp/cpu_acoustic.cpp):
+    the single-core program as an -O3 -DNDEBUG build and with the reference's
+    own flags (-O3, asserts on; singlecore/build/makefile:44 adds -march=native,
+    left out so the binary runs on any host), and the MPI program on `threads`
+    ranks where the image has MPI.  Updates/s from the "loop time" each prints."""
+    import shutil
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    out = {}
+    runs = [("singlecore_ndebug", os.path.join(ref, "singlecore_cli_f32"), 3000, 1, None),
+            ("singlecore_refflags", os.path.join(ref, "singlecore_cli_f32_refflags"), 500, 1, None)]
+    mpiexec = shutil.which("mpiexec") or ("/opt/conda/bin/mpiexec" if os.access("/opt/conda/bin/mpiexec", os.X_OK)
+                                          else None)
+    if mpiexec:
+        runs.append(("mpi_refflags", os.path.join(ref, "mpi_acoustic_f32_refflags"), 2000, threads, mpiexec))
+    for name, exe, nt, np_, launcher in runs:
+        if not os.access(exe, os.X_OK):
+            out[name] = {"value": None, "note": "not built (oracle/ref_build.sh needs the reference tree%s)" %
+                         (" and MPI" if launcher else "")}
+            continue
+        cmd = [exe, "-Nt", str(nt)]
+        if launcher:
+            dims = _grid_dims(np_)
+            cmd = [launcher, "-n", str(np_), exe, "-Nt", str(nt), "-procs", "%d %d" % dims]
+        secs = _reference_loop_seconds(cmd, timeout_s)
+        if secs is None or secs <= 0:
+            out[name] = {"value": None, "note": "did not run here: %s" % " ".join(os.path.basename(c) for c in cmd)}
+            continue
+        out[name] = {"value": 600 * 600 * nt / secs, "unit": "grid-point updates/s", "cores": np_,
+                     "kind": "reference", "sample": "%s, 600x600 periodic FP32, %d steps, loop %.2f s" %
+                     (os.path.basename(exe), nt, secs)}
+    if not mpiexec:
+        out["mpi_refflags"] = {"value": None, "note": "no MPI launcher on this host"}
+    return out
+
+
+def gpu_reference_case(torch, T, nt=2000):
+    """This solver on the reference's own acoustic CPU case (600x600 periodic,
+    FP32, dx = 0.008, dt = 1e-4, its stencil and energy forms), for a
+    like-for-like ratio with reference_cpu_samples."""
+    cfg = T.config(physics=T.ACOUSTIC, mx=600, my=600, dx=(1.0 / (5.0 * 2.5)) / 10, num_dx=0, den_dx=0, dt=1e-4,
+                   bc="PPPP", precision=T.FP32, cpst="0", stencil_form=1, energy_form=1, nt_max=nt + 100)
+    s = T.Solver(cfg)
+    s.set_source_ricker(T.GRID_MM, 200, 200, 5.0, 0.25)
+    s.set_receivers([T.VX, T.VY, T.P], [400, 400, 400], [400, 400, 400])
+    t, _, _ = timed_steps(torch, s, nt, 100)
+    s.close()
+    return {"value": 600 * 600 * nt / t, "unit": "grid-point updates/s", "ms_per_step": t / nt * 1e3,
+            "sample": "600x600 periodic acoustic FP32 (the reference CPU case), %d steps, energy and 3 receivers "
+                      "recorded every step" % nt}
 
 
 def c4_batch(args):
@@ -314,6 +358,9 @@ def main():
                     help="material coefficient storage: 0 FP32, 1 the 16-bit type, 2 one byte per value with a "
                     "per-row affine map (default)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32, FP64 and variant runs")
+    ap.add_argument("--accuracy-steps", type=int, default=2000,
+                    help="step count at which every run is compared with FP64 (untimed beyond the timed range; "
+                    "the configuration's 2000 steps by default)")
     ap.add_argument("--variants", default="bf16:K8:1,bf16:0:2,fp16:K8:2",
                     help="comma-separated storage:cpst[:coef16] triples (coef16 default 1) timed and checked "
                     "against FP64 beside the main run, on the same material")
@@ -348,7 +395,11 @@ def main():
     nt = K + Wn
     prec = T.PRECISION[args.precision]
 
-    s = make_solver(T, W, m, prec, args.cpst, args.coef16, nt)
+    # every run advances to the same step count (the configuration's 2000 steps
+    # unless the timed range is longer) before fields and traces are compared
+    n_acc = max(args.accuracy_steps, nt)
+
+    s = make_solver(T, W, m, prec, args.cpst, args.coef16, n_acc)
     t_gpu, wall, prof = timed_steps(torch, s, K, Wn, profile=not args.no_profile)
     value = m * m * K / t_gpu
     B = algorithmic_bytes(prec, args.cpst, args.coef16)
@@ -394,51 +445,85 @@ def main():
         "roofline": roof,
     }
     if not args.no_compare:
-        fields_lp = {f: s.get_field(f) for f in s.fields}
-        rec_lp = s.receivers()
-        # same (quantized) material as the 16-bit run, so the three runs differ only in storage
-        s32 = make_solver(T, W, m, T.FP32, "0", 0, nt, model_from=s)
-        t32, _, _ = timed_steps(torch, s32, K, Wn)
-        out["fp32_value"] = m * m * K / t32
-        out["speedup_vs_fp32"] = out["value"] / out["fp32_value"]
-        fields32 = {f: s32.get_field(f) for f in s32.fields}
-        rec32 = s32.receivers()
-        s32.close()
-        s64 = make_solver(T, W, m, T.FP64, "0", 0, nt, model_from=s)
-        out["material_max_rel_err"] = s64.material_err
-        s64.step(nt)
-        f64 = {f: s64.get_field(f) for f in s64.fields}
-        rec64 = s64.receivers()
-        s64.close()
+        def finish(sv):
+            """Advance to n_acc steps (untimed) and return the compensated
+            fields and the receiver traces."""
+            sv.step(n_acc - sv.current_step())
+            sv.synchronize()
+            return {f: sv.get_field_compensated(f) for f in sv.fields}, sv.receivers()
 
         def rl2(a, b):
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-        out["rel_l2_vs_fp64"] = {
-            "storage": {"fields": max(rl2(fields_lp[f], f64[f]) for f in f64), "receivers": rl2(rec_lp, rec64)},
-            "fp32": {"fields": max(rl2(fields32[f], f64[f]) for f in f64), "receivers": rl2(rec32, rec64)},
+
+        def errs(run, ref):
+            (fa, ra), (fb, rb) = run, ref
+            return {"fields": max(rl2(fa[f], fb[f]) for f in fb), "receivers": rl2(ra, rb)}
+
+        main_run = finish(s)
+        # FP64 on the same (quantized) material: the error of the state storage alone
+        s64 = make_solver(T, W, m, T.FP64, "0", 0, n_acc, model_from=s)
+        material_err = s64.material_err
+        ref_q = finish(s64)
+        s64.close()
+        # FP64 on the exact material: the error of storage and coefficient quantization together
+        s64 = make_solver(T, W, m, T.FP64, "0", 0, n_acc)
+        ref_x = finish(s64)
+        s64.close()
+        s32 = make_solver(T, W, m, T.FP32, "0", 0, n_acc, model_from=s)
+        t32, _, _ = timed_steps(torch, s32, K, Wn)
+        out["fp32_value"] = m * m * K / t32
+        out["speedup_vs_fp32"] = out["value"] / out["fp32_value"]
+        r32 = finish(s32)
+        s32.close()
+        out["material_max_rel_err"] = material_err
+        out["accuracy"] = {
+            "steps": n_acc,
+            "fields": "stored value + decoded compensation (tw_get_field_compensated), max over the 5 fields of "
+                      "the relative L2 error",
+            "receivers": "relative L2 error of all 256 traces over all steps",
+            "reference": "FP64 run of the same scheme; 'same_material' uses the quantized coefficients of the "
+                         "main run (state storage error), 'exact_material' the exact FP64 interpolation "
+                         "(storage + coefficient quantization)",
+            "target": 1e-4,
         }
-        # other storage variants on the same material and steps: speed and error side by side
+        out["rel_l2_vs_fp64"] = {
+            "storage": dict(errs(main_run, ref_q), exact_material=errs(main_run, ref_x)),
+            "fp32": dict(errs(r32, ref_q), exact_material=errs(r32, ref_x)),
+        }
+        del main_run, r32
+        # other storage variants: speed and error side by side
         out["variants"] = []
         for spec in filter(None, args.variants.split(",")):
             vp, vc, vk = (spec.split(":") + ["1"])[:3]
             vk = int(vk)
-            sv = make_solver(T, W, m, T.PRECISION[vp], vc, vk, nt, model_from=s)
+            if vk == args.coef16:
+                sv = make_solver(T, W, m, T.PRECISION[vp], vc, vk, n_acc, model_from=s)
+            else:  # other coefficient storage: its own quantization of the model
+                sv = make_solver(T, W, m, T.PRECISION[vp], vc, vk, n_acc)
             tv, _, _ = timed_steps(torch, sv, K, Wn)
-            fv = {f: sv.get_field(f) for f in sv.fields}
-            rv = sv.receivers()
+            run = finish(sv)
+            # its own quantized material in FP64, where it differs from the main run's
+            if vk == args.coef16:
+                e_same = errs(run, ref_q)
+            else:
+                s64 = make_solver(T, W, m, T.FP64, "0", 0, n_acc, model_from=sv)
+                e_same = errs(run, finish(s64))
+                s64.close()
             sv.close()
             val = m * m * K / tv
             out["variants"].append({
                 "storage": vp, "cpst": vc, "coef16": vk, "value": val, "ms_per_step": tv / K * 1e3,
                 "speedup_vs_fp32": val / out["fp32_value"],
                 "bytes_per_update": algorithmic_bytes(T.PRECISION[vp], vc, vk),
-                "rel_l2_vs_fp64": {"fields": max(rl2(fv[f], f64[f]) for f in f64), "receivers": rl2(rv, rec64)}})
+                "rel_l2_vs_fp64": dict(e_same, exact_material=errs(run, ref_x))})
+            del run
     s.close()
     if not args.no_cpu:
+        threads = cpu_threads()
         out["cpu_baseline"] = cpu_baseline(args.cpu_size or m)
-        ref = reference_cpu_sample()
-        if ref is not None:
-            out["cpu_baseline"]["reference_solver"] = ref
+        out["cpu_baseline"]["host_cpus"] = os.cpu_count()
+        out["cpu_baseline"]["reference_acoustic"] = reference_cpu_samples(threads)
+        out["cpu_baseline"]["gpu_on_reference_acoustic_case"] = gpu_reference_case(torch, T)
     print(json.dumps(out))
 
 

@@ -60,10 +60,14 @@ enum tw_precision { TW_FP64 = 0, TW_FP32 = 1, TW_BF16 = 2, TW_FP16 = 3 };
  *   TW_CPST_K   16-bit storage only: FP32 sum, the rounding residual of the
  *               store is kept in a 16-bit companion array (Kahan-style)
  *   TW_CPST_K8  as K with the residual kept at 8 bits: a code q in
- *               [-127, 127] per value, residual = q * ulp(stored value) / 256 */
+ *               [-127, 127] per value, residual = q * ulp(stored value) / 256
+ *   TW_CPST_K8S as K8, and every stencil, SAT and boundary-value input reads
+ *               the stored value plus its decoded residual (the compensated
+ *               value) instead of the stored value alone */
 enum tw_cpst {
   TW_CPST_0 = 0, TW_CPST_3R = 1, TW_CPST_M3R = 2, TW_CPST_6R = 3,
-  TW_CPST_3C = 4, TW_CPST_M3C = 5, TW_CPST_6C = 6, TW_CPST_K = 7, TW_CPST_K8 = 8
+  TW_CPST_3C = 4, TW_CPST_M3C = 5, TW_CPST_6C = 6, TW_CPST_K = 7, TW_CPST_K8 = 8,
+  TW_CPST_K8S = 9
 };
 enum tw_field { TW_VX = 0, TW_VY = 1, TW_SXX = 2, TW_SYY = 3, TW_SXY = 4, TW_P = 5 };
 enum tw_grid  { TW_GRID_NN = 0, TW_GRID_MN = 1, TW_GRID_NM = 2, TW_GRID_MM = 3 };
@@ -209,6 +213,14 @@ int tw_set_field(tw_solver* s, int32_t field, const double* in);
 /* Companion array (R or C compensation, or the Kahan residual) of a field;
  * for TW_CPST_K8 the residual codes as integers (-127..127). */
 int tw_get_companion(tw_solver* s, int32_t field, double* out);
+/* The field's value including what its compensation carries: the FP64 sum of
+ * the stored value and its R, C or K companion, or of the stored value and
+ * its decoded K8 residual (q * ulp(value) / 256); the stored value for
+ * TW_CPST_0.  This is the value the compensated update works with (the
+ * reference keeps S and its error term side by side,
+ * cuda/src/cuda/grid_update.cut:62-124), and the one to compare with a
+ * higher-precision run. */
+int tw_get_field_compensated(tw_solver* s, int32_t field, double* out);
 
 /* Zero all fields, companions, records and the step counter. */
 int tw_reset(tw_solver* s);

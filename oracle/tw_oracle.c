@@ -21,6 +21,9 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+/* the 8-bit residual variants: K8, and K8S whose stencils read the residual too */
+#define OR_K8(cpst) ((cpst) == TW_CPST_K8 || (cpst) == TW_CPST_K8S)
 #endif
 
 /* ---- operator constants: cuda/src/cpp/namespace_forward.hpp:65-156 ---- */
@@ -264,10 +267,10 @@ static int cfg_check(const tw_config* c) {
   for (int a = 0; a < 2; a++)
     if ((c->bc[2 * a] == 'P') != (c->bc[2 * a + 1] == 'P')) { or_seterr("periodic must pair"); return TW_EINVAL; }
   if (c->precision < TW_FP64 || c->precision > TW_FP16) { or_seterr("precision"); return TW_EINVAL; }
-  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K8) { or_seterr("cpst"); return TW_EINVAL; }
+  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K8S) { or_seterr("cpst"); return TW_EINVAL; }
   int ref = c->precision == TW_FP64 || c->precision == TW_FP32;
-  if (ref && (c->cpst == TW_CPST_K || c->cpst == TW_CPST_K8)) { or_seterr("cpst K/K8 needs 16-bit storage"); return TW_EINVAL; }
-  if (!ref && c->cpst != TW_CPST_0 && c->cpst != TW_CPST_K && c->cpst != TW_CPST_K8) {
+  if (ref && (c->cpst == TW_CPST_K || OR_K8(c->cpst))) { or_seterr("cpst K/K8/K8S needs 16-bit storage"); return TW_EINVAL; }
+  if (!ref && c->cpst != TW_CPST_0 && c->cpst != TW_CPST_K && !OR_K8(c->cpst)) {
     or_seterr("16-bit storage supports cpst 0, K or K8");
     return TW_EINVAL;
   }
@@ -684,7 +687,7 @@ int or_update_points(int precision, int cpst, long n, double* s, double* r, doub
 }
 
 int or_store16_points(int precision, int cpst, long n, const double* t, double* hi, double* res) {
-  if (cpst != TW_CPST_0 && cpst != TW_CPST_K && cpst != TW_CPST_K8) return TW_EINVAL;
+  if (cpst != TW_CPST_0 && cpst != TW_CPST_K && !OR_K8(cpst)) return TW_EINVAL;
   if (precision == TW_BF16) bf16_store_points(cpst, n, t, hi, res);
   else if (precision == TW_FP16) f16_store_points(cpst, n, t, hi, res);
   else return TW_EINVAL;

@@ -218,6 +218,16 @@ static inline CREAL FN(val)(const FN(fld) * F, size_t i, int kmode) {
   return v;
 }
 
+/* a value as the stencils, SAT terms and boundary values read it: the stored
+ * value, or for K8S the stored value plus its decoded residual (FP32 sum) */
+static inline CREAL FN(pv)(const FN(fld) * F, size_t i, int rf) {
+#if !REFMODE
+  if (rf) return S2C(F->s[i]) + FN(k8_dec)(F->r[i], F->s[i]);
+#endif
+  (void)rf;
+  return S2C(F->s[i]);
+}
+
 /* ---------- right-hand side accumulation ---------- */
 
 /* start of a half step: acc = R (reference variants; R is zero after the
@@ -228,7 +238,7 @@ static void FN(acc_init)(FN(st_t) * S, int f) {
 #if REFMODE
   for (size_t i = 0; i < n; i++) F->acc[i] = F->r[i];
 #else
-  int k = S->cm->cfg.cpst == TW_CPST_K, k8 = S->cm->cfg.cpst == TW_CPST_K8;
+  int k = S->cm->cfg.cpst == TW_CPST_K, k8 = OR_K8(S->cm->cfg.cpst);
   for (size_t i = 0; i < n; i++)
     F->acc[i] = k ? S2C(F->r[i]) : (k8 ? FN(k8_dec)(F->r[i], F->s[i]) : (CREAL)0);
 #endif
@@ -245,6 +255,7 @@ static void FN(dx_term)(FN(st_t) * S, int f, int pf, int coefk, int out_t) {
   const CREAL* cp = S->coef[coefk];
   FN(taps)* T = &S->tx[out_t];
   int rows = F->rows, cols = F->cols;
+  const int rf = S->cm->cfg.cpst == TW_CPST_K8S;
 #pragma omp parallel for schedule(static)
   for (int ix = 0; ix < rows; ix++) {
     int nt = T->ntap[ix];
@@ -254,7 +265,7 @@ static void FN(dx_term)(FN(st_t) * S, int f, int pf, int coefk, int out_t) {
     const CREAL* pr = cp + (size_t)ix * cols;
     for (int iy = 0; iy < cols; iy++) {
       CREAL v = 0;
-      for (int k = 0; k < nt; k++) v = v + S2C(P->s[(size_t)id[k] * P->cols + iy]) * cf[k];
+      for (int k = 0; k < nt; k++) v = v + FN(pv)(P, (size_t)id[k] * P->cols + iy, rf) * cf[k];
       a[iy] = a[iy] + v * pr[iy];
     }
   }
@@ -266,17 +277,18 @@ static void FN(dy_term)(FN(st_t) * S, int f, int pf, int coefk, int out_t) {
   const CREAL* cp = S->coef[coefk];
   FN(taps)* T = &S->ty[out_t];
   int rows = F->rows, cols = F->cols;
+  const int rf = S->cm->cfg.cpst == TW_CPST_K8S;
 #pragma omp parallel for schedule(static)
   for (int ix = 0; ix < rows; ix++) {
     CREAL* a = F->acc + (size_t)ix * cols;
     const CREAL* pr = cp + (size_t)ix * cols;
-    const SREAL* ps = P->s + (size_t)ix * P->cols;
+    const size_t p0 = (size_t)ix * P->cols;
     for (int iy = 0; iy < cols; iy++) {
       int nt = T->ntap[iy];
       const int* id = T->idx + 5 * iy;
       const CREAL* cf = T->cf + 5 * iy;
       CREAL v = 0;
-      for (int k = 0; k < nt; k++) v = v + S2C(ps[id[k]]) * cf[k];
+      for (int k = 0; k < nt; k++) v = v + FN(pv)(P, p0 + id[k], rf) * cf[k];
       a[iy] = a[iy] + v * pr[iy];
     }
   }
@@ -291,6 +303,7 @@ static void FN(mm_term)(FN(st_t) * S, int pf, int dir) {
   const CREAL* tmu = S->coef[TW_C_TWOMU];
   int rows = A->rows, cols = A->cols;
   FN(taps)* T = dir == 0 ? &S->tx[0] : &S->ty[0];
+  const int rf = S->cm->cfg.cpst == TW_CPST_K8S;
 #pragma omp parallel for schedule(static)
   for (int ix = 0; ix < rows; ix++) {
     for (int iy = 0; iy < cols; iy++) {
@@ -300,9 +313,9 @@ static void FN(mm_term)(FN(st_t) * S, int pf, int dir) {
       const CREAL* cf = T->cf + 5 * i;
       CREAL v = 0;
       if (dir == 0)
-        for (int k = 0; k < nt; k++) v = v + S2C(P->s[(size_t)id[k] * P->cols + iy]) * cf[k];
+        for (int k = 0; k < nt; k++) v = v + FN(pv)(P, (size_t)id[k] * P->cols + iy, rf) * cf[k];
       else
-        for (int k = 0; k < nt; k++) v = v + S2C(P->s[(size_t)ix * P->cols + id[k]]) * cf[k];
+        for (int k = 0; k < nt; k++) v = v + FN(pv)(P, (size_t)ix * P->cols + id[k], rf) * cf[k];
       size_t q = (size_t)ix * cols + iy;
       A->acc[q] = A->acc[q] + v * lam[q];
       B->acc[q] = B->acc[q] + v * lam[q];
@@ -325,6 +338,7 @@ static void FN(sat)(FN(st_t) * S, int f, int pf, int coefk, int axis, int out_t)
   FN(fld)* F = &S->f[f];
   FN(fld)* P = &S->f[pf];
   const CREAL* cp = S->coef[coefk];
+  const int rf = c->cpst == TW_CPST_K8S;
   for (int side = 0; side < 2; side++) {
     char bc = c->bc[2 * axis + side];
     if (bc == 'P') continue;
@@ -343,7 +357,7 @@ static void FN(sat)(FN(st_t) * S, int f, int pf, int coefk, int axis, int out_t)
         int pa = side == 0 ? w : np_along - plen + w;
         size_t q = axis == 0 ? (size_t)pa * P->cols + j : (size_t)j * P->cols + pa;
         CREAL pw = out_t ? proj[w] : (CREAL)1;
-        send = send + pw * S2C(P->s[q]);
+        send = send + pw * FN(pv)(P, q, rf);
       }
       CREAL recv = 0;
       if (chr) {
@@ -351,13 +365,13 @@ static void FN(sat)(FN(st_t) * S, int f, int pf, int coefk, int axis, int out_t)
         if (out_t) {
           int a = side == 0 ? 0 : n_along - 1;
           size_t q = axis == 0 ? (size_t)a * F->cols + j : (size_t)j * F->cols + a;
-          u = S2C(F->s[q]);
+          u = FN(pv)(F, q, rf);
         } else {
           const CREAL* pm = side == 0 ? S->projM_L : S->projM_R;
           for (int w = 0; w < 3; w++) {
             int a = side == 0 ? w : n_along - 3 + w;
             size_t q = axis == 0 ? (size_t)a * F->cols + j : (size_t)j * F->cols + a;
-            u = u + pm[w] * S2C(F->s[q]);
+            u = u + pm[w] * FN(pv)(F, q, rf);
           }
         }
         CREAL zu = z[j] * u;
@@ -418,7 +432,7 @@ static inline void FN(upd_point16)(int cpst, SREAL* ps, SREAL* pr, CREAL acc) {
   SREAL hi = C2S(t);
   *ps = hi;
   if (cpst == TW_CPST_K) *pr = C2S(t - S2C(hi));
-  if (cpst == TW_CPST_K8) *pr = FN(k8_enc)(t - S2C(hi), hi);
+  if (OR_K8(cpst)) *pr = FN(k8_enc)(t - S2C(hi), hi);
 }
 #endif
 
@@ -462,7 +476,7 @@ static void FN(apply_source)(FN(st_t) * S, int half) {
       SREAL hi = C2S(vf);
       F->s[q] = hi;
       F->r[q] = C2S(vf - S2C(hi));
-    } else if (cm->cfg.cpst == TW_CPST_K8) {
+    } else if (OR_K8(cm->cfg.cpst)) {
       double v = (double)S2C(F->s[q]) + (double)FN(k8_dec)(F->r[q], F->s[q]) + inc;
       CREAL vf = (CREAL)v;
       SREAL hi = C2S(vf);
@@ -480,7 +494,7 @@ static double FN(energy_field)(FN(st_t) * S, int f) {
   or_common* cm = S->cm;
   const tw_config* c = &cm->cfg;
   FN(fld)* F = &S->f[f];
-  int kmode = c->cpst == TW_CPST_K ? 1 : (c->cpst == TW_CPST_K8 ? 2 : 0);
+  int kmode = c->cpst == TW_CPST_K ? 1 : (OR_K8(c->cpst) ? 2 : 0);
   int g = or_field_grid(c->physics, f);
   int tx = (g == TW_GRID_NN || g == TW_GRID_NM), ty = (g == TW_GRID_NN || g == TW_GRID_MN);
   double E = 0.0;
@@ -524,7 +538,7 @@ static double FN(energy_mm)(FN(st_t) * S) {
   const tw_config* c = &cm->cfg;
   FN(fld)* A = &S->f[TW_SXX];
   FN(fld)* B = &S->f[TW_SYY];
-  int kmode = c->cpst == TW_CPST_K ? 1 : (c->cpst == TW_CPST_K8 ? 2 : 0);
+  int kmode = c->cpst == TW_CPST_K ? 1 : (OR_K8(c->cpst) ? 2 : 0);
   const CREAL* lam = S->coef[TW_C_LAMBDA];
   const CREAL* tmu = S->coef[TW_C_TWOMU];
   double E = 0.0;
@@ -618,7 +632,7 @@ static void FN(one_step)(void* p) {
     cm->ep1[it] = cm->ep_prev;
   }
   if (it < c->nt_max) {
-    int kmode = c->cpst == TW_CPST_K ? 1 : (c->cpst == TW_CPST_K8 ? 2 : 0);
+    int kmode = c->cpst == TW_CPST_K ? 1 : (OR_K8(c->cpst) ? 2 : 0);
     for (int r = 0; r < cm->nrcv; r++) {
       FN(fld)* F = &S->f[cm->rf[r]];
       size_t q = (size_t)cm->rix[r] * F->cols + cm->riy[r];

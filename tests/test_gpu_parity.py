@@ -248,6 +248,32 @@ def test_gpu_against_reference_golden(prec, name, cpst):
     s.close()
 
 
+@pytest.mark.parametrize("precision,cpst", [("bf16", "K8"), ("fp16", "K8"), ("bf16", "K"), ("fp32", "6R"),
+                                            ("fp32", "3C"), ("bf16", "0")])
+def test_field_compensated(precision, cpst):
+    """tw_get_field_compensated = stored value + its compensation, computed here
+    independently from the oracle's state and companion (K8: code * 2^(max(e, 1)
+    - bias) via the oracle's decoder; K, R, C: the companion value)."""
+    import oracle as O
+    s, o = C.make_pair(physics="elastic", mx=40, my=48, bc="FCCC", precision=precision, cpst=cpst, nt=30)
+    C.random_material(s, o, 5)
+    C.random_fields(s, o, 6)
+    for h in (s, o):
+        h.set_source_ricker(T.GRID_NM, 10, 20, 25.0, 0.01)
+        h.step(30)
+    for f in s.fields:
+        v, q = o.get_field(f), o.get_companion(f)
+        if cpst == "K8":
+            want = v + O.k8_decode_points(C.PREC[precision], v, q).reshape(v.shape)
+        else:
+            want = v + q
+        got = s.get_field_compensated(f)
+        np.testing.assert_array_equal(got, want, err_msg=T.FIELD_NAMES[f])
+        if cpst != "0":
+            assert np.any(got != s.get_field(f))
+    s.close()
+
+
 def test_model_interpolation_parity():
     """tw_set_model (host) and the oracle's restatement give identical coefficients."""
     s, o = C.make_pair(physics="elastic", mx=40, my=48, bc="FFPP", precision="fp32", nt=10, num_dx=1, den_dx=100)

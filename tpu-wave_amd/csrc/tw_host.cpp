@@ -1158,6 +1158,32 @@ int tw_get_companion(tw_solver* s, int32_t field, double* out) {
   return copy_field(s, field, cmode(s->cfg.cpst) ? 2 : 1, out, nullptr);
 }
 
+int tw_get_field_compensated(tw_solver* s, int32_t field, double* out) {
+  if (!s || !out) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  int e = tw_get_field(s, field, out);
+  if (e || s->cfg.cpst == TW_CPST_0) return e;
+  int rows, cols;
+  shape_of(&s->cfg, field_grid(field), &rows, &cols);
+  std::vector<double> comp((size_t)rows * cols);
+  if ((e = tw_get_companion(s, field, comp.data()))) return e;
+  if (s->cfg.cpst == TW_CPST_K8) {
+    // code q -> q * 2^(max(e, 1) - bias), e the stored value's exponent field
+    // (BF16: bias 142, exponent bias 127; FP16: bias 33, exponent bias 15)
+    const int bias = s->cfg.precision == TW_BF16 ? 142 : 33, ebias = s->cfg.precision == TW_BF16 ? 127 : 15;
+    for (size_t i = 0; i < comp.size(); i++) {
+      int ex = 0;
+      if (out[i] != 0.0 && std::isfinite(out[i])) {
+        (void)std::frexp(out[i], &ex);
+        ex = ex - 1 + ebias;  // biased exponent of a normal value
+      }
+      out[i] += std::ldexp(comp[i], std::max(ex, 1) - bias);
+    }
+  } else {
+    for (size_t i = 0; i < comp.size(); i++) out[i] += comp[i];
+  }
+  return TW_OK;
+}
+
 int tw_reset(tw_solver* s) {
   if (!s) return fail(nullptr, TW_EINVAL, "null handle");
   (void)hipSetDevice(s->cfg.device);

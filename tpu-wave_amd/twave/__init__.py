@@ -129,6 +129,7 @@ _SIGS = [
     ("tw_get_field", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
     ("tw_set_field", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
     ("tw_get_companion", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    ("tw_get_field_compensated", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
     ("tw_reset", ctypes.c_int, [_vp]),
     ("tw_checkpoint_size", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     ("tw_checkpoint_save", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
@@ -368,6 +369,13 @@ class Solver:
         if v.shape != self.shape(field):
             raise ValueError("field shape %s, expected %s" % (v.shape, self.shape(field)))
         self._chk(lib().tw_set_field(self.h, field, _d(v)))
+
+    def get_field_compensated(self, field):
+        """Stored value plus its compensation (decoded K8 residual, K/R/C
+        companion): tw_get_field_compensated."""
+        out = np.zeros(self.shape(field))
+        self._chk(lib().tw_get_field_compensated(self.h, field, _d(out)))
+        return out
 
     def get_companion(self, field):
         out = np.zeros(self.shape(field))
