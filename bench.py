@@ -40,7 +40,7 @@ METRIC = "grid-pt updates/s & % HBM peak; BF16-store vs FP32 at ≤1e-4 rel L2 v
 def comp_bytes(s, cpst):
     """Bytes of the stored compensation per value: a storage-type companion
     array, one byte for the K8 residual code, none for cpst 0."""
-    if cpst == "K8":
+    if cpst in ("K8", "K8S"):
         return 1
     return s if cpst in ("3R", "-3R", "6R", "3C", "-3C", "6C", "K") else 0
 

@@ -336,7 +336,7 @@ int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* en
  * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:755-772):
  * <root>/<elastic|acoustic>/<medium>/device/grid_size_<num>_<den>/Nt_<Nt>/
  * <fp64|fp32|bf16|fp16>/dt_<dt as %16.15e with the '.' as 'p'>/<cpst>, cpst as
- * 0_0, 3_R, -3_R, 6_R, 3_C, -3_C, 6_C, K or K8.  Writes at most cap bytes. */
+ * 0_0, 3_R, -3_R, 6_R, 3_C, -3_C, 6_C, K, K8 or K8S.  Writes at most cap bytes. */
 int tw_result_folder(const tw_input* in, const tw_config* cfg, const char* root, char* out, int32_t cap);
 
 /* Data misfit of the recorded traces against data [n][nt] (the solver's

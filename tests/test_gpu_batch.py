@@ -45,7 +45,8 @@ def assert_same(a, b):
             np.testing.assert_array_equal(x.get_companion(f), y.get_companion(f), err_msg=T.FIELD_NAMES[f])
 
 
-@pytest.mark.parametrize("precision,cpst", [("fp32", "0"), ("bf16", "K"), ("bf16", "0"), ("fp64", "6R"), ("bf16", "K8")])
+@pytest.mark.parametrize("precision,cpst", [("fp32", "0"), ("bf16", "K"), ("bf16", "0"), ("fp64", "6R"), ("bf16", "K8"),
+                                            ("fp16", "K8S")])
 def test_batch_equals_solo_acoustic(precision, cpst):
     print("case batch acoustic", precision, cpst, flush=True)
     batched, solo = pairs("acoustic", precision, cpst)
@@ -103,7 +104,8 @@ def oracle_twin(physics, precision, cpst, mx, my, bc, seed, nt=200, coef16=0):
 
 
 @pytest.mark.parametrize("physics,precision,cpst,coef16", [("acoustic", "bf16", "K8", 1), ("acoustic", "bf16", "0", 2),
-                                                           ("elastic", "bf16", "K8", 2), ("acoustic", "fp32", "6R", 0)])
+                                                           ("elastic", "bf16", "K8", 2), ("acoustic", "fp32", "6R", 0),
+                                                           ("elastic", "fp16", "K8S", 2), ("acoustic", "bf16", "K8S", 1)])
 def test_batch_equals_oracle(physics, precision, cpst, coef16):
     """Batched launches against the CPU oracle directly (not only against the
     same problems stepped alone): every field and companion bitwise after a

@@ -115,8 +115,23 @@ def test_elastic_sbp_k8(bc, precision):
     run_case("elastic", bc, precision, "K8", coef16=1, mx=40, my=48, nt=30)
 
 
+@pytest.mark.parametrize("physics,bc,precision,coef16,mx,my", [
+    ("elastic", "FFPP", "bf16", 1, 40, 48), ("elastic", "FCCC", "fp16", 1, 40, 48), ("elastic", "FFFF", "bf16", 0, 36, 44),
+    ("elastic", "CCCC", "fp16", 0, 36, 44), ("elastic", "PPPP", "bf16", 1, 40, 48), ("elastic", "CFFC", "bf16", 1, 36, 44),
+    ("elastic", "FCFC", "bf16", 1, 21, 531), ("elastic", "PPCC", "fp16", 1, 21, 531),
+    ("acoustic", "FCPP", "fp16", 1, 40, 48), ("acoustic", "CCCC", "bf16", 0, 40, 48),
+    ("acoustic", "PPPP", "bf16", 1, 17, 517)])
+def test_k8s_residual_fed(physics, bc, precision, coef16, mx, my):
+    """K8S: stencils, SAT terms and characteristic boundary values read the
+    stored value plus its decoded residual; fields, residual codes, traces
+    bitwise equal to the oracle's K8S, energies to 1e-12."""
+    run_case(physics, bc, precision, "K8S", coef16, mx=mx, my=my, nt=30 if my < 256 else 12,
+             stencil_form=1 if physics == "acoustic" else 0)
+
+
 @pytest.mark.parametrize("physics,bc,precision,cpst,mx,my", [
     ("elastic", "FCCC", "bf16", "K8", 40, 48), ("elastic", "FFPP", "fp16", "K8", 40, 48),
+    ("elastic", "FCCC", "bf16", "K8S", 40, 48), ("acoustic", "CCCC", "fp16", "K8S", 40, 48),
     ("elastic", "PPPP", "bf16", "0", 40, 48), ("elastic", "FCFC", "bf16", "K8", 21, 531),
     ("acoustic", "CCCC", "bf16", "0", 40, 48), ("acoustic", "FCPP", "fp16", "K8", 40, 48),
     ("acoustic", "PPCC", "bf16", "K", 17, 517)])
@@ -249,7 +264,7 @@ def test_gpu_against_reference_golden(prec, name, cpst):
 
 
 @pytest.mark.parametrize("precision,cpst", [("bf16", "K8"), ("fp16", "K8"), ("bf16", "K"), ("fp32", "6R"),
-                                            ("fp32", "3C"), ("bf16", "0")])
+                                            ("fp32", "3C"), ("bf16", "0"), ("fp16", "K8S")])
 def test_field_compensated(precision, cpst):
     """tw_get_field_compensated = stored value + its compensation, computed here
     independently from the oracle's state and companion (K8: code * 2^(max(e, 1)
@@ -263,7 +278,7 @@ def test_field_compensated(precision, cpst):
         h.step(30)
     for f in s.fields:
         v, q = o.get_field(f), o.get_companion(f)
-        if cpst == "K8":
+        if cpst in ("K8", "K8S"):
             want = v + O.k8_decode_points(C.PREC[precision], v, q).reshape(v.shape)
         else:
             want = v + q

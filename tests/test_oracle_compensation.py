@@ -175,3 +175,54 @@ def test_k_residual_closed_form(prec):
     hi0, r0 = O.store16_points(prec, "0", t)
     np.testing.assert_array_equal(hi0, hi)
     assert not r0.any()
+
+
+def _state_with_residuals(o, prec, fields, rng):
+    """Random 16-bit values with random residual codes on `fields`; returns the
+    compensated values hi + decode(code), exact in binary32."""
+    vals = {}
+    for f in fields:
+        shape = o.shape(f)
+        hi = _round16(prec, rng.standard_normal(shape))
+        o.set_field(f, hi)
+        vals[f] = hi
+    return vals
+
+
+@pytest.mark.parametrize("prec", [O.BF16, O.FP16])
+@pytest.mark.parametrize("bc", ["FCCC", "CFPP", "PPPP"])
+def test_k8s_stencils_read_the_residual(prec, bc):
+    """K8S: every stencil, SAT and boundary-value input reads the stored value
+    plus its decoded residual.  Start from stress fields with residuals (the
+    K8 state after a first step from stress-only initial values, which the
+    oracle produces itself) and compare one step of K8S and K8 with an FP32
+    run started from the exact compensated values: K8S stays within the
+    codes' resolution of FP32 everywhere (closure rows and SAT boundary rows
+    included), K8, whose stencils drop the residuals, is far off."""
+    m = 24
+    kw = dict(physics=O.ELASTIC, mx=m, my=m + 4, bc=bc, nt_max=4, dt=0.3 / 125 / (2 * 2 ** 0.5))
+    rng = np.random.default_rng(3)
+    runs = {}
+    for cp in ("K8S", "K8"):
+        o = O.Oracle(O.make_config(precision=prec, cpst=cp, **kw))
+        rng = np.random.default_rng(3)
+        _state_with_residuals(o, prec, [O.SXX, O.SYY, O.SXY], rng)
+        o.step(1)  # creates residual codes on every field
+        comp = {f: o.get_field(f) + O.k8_decode_points(prec, o.get_field(f), o.get_companion(f)).reshape(o.shape(f))
+                for f in (O.VX, O.VY, O.SXX, O.SYY, O.SXY)}
+        f32 = O.Oracle(O.make_config(precision=O.FP32, cpst="0", **kw))
+        for f, v in comp.items():
+            f32.set_field(f, v)
+            np.testing.assert_array_equal(f32.get_field(f), v)  # exact in binary32
+        o.step(1)
+        f32.step(1)
+        runs[cp] = {f: (o.get_field(f) + O.k8_decode_points(prec, o.get_field(f), o.get_companion(f)).reshape(
+            o.shape(f))) - f32.get_field(f) for f in comp}
+        scale = {f: np.abs(f32.get_field(f)).max() for f in comp}
+        assert all(np.any(o.get_companion(f)) for f in comp)
+    ulp = 2.0 ** -(7 if prec == O.BF16 else 10)
+    for f in runs["K8S"]:
+        e8s = np.abs(runs["K8S"][f]).max() / scale[f]
+        e8 = np.abs(runs["K8"][f]).max() / scale[f]
+        assert e8s < ulp / 64, (f, e8s)
+        assert e8 > 8 * e8s, (f, e8, e8s)

@@ -87,7 +87,8 @@ bool coef_in(int physics, int k) {
 }
 bool is_velocity(int f) { return f == TW_VX || f == TW_VY; }
 bool ref_precision(int p) { return p == TW_FP64 || p == TW_FP32; }
-bool rmode(int c) { return c == TW_CPST_3R || c == TW_CPST_M3R || c == TW_CPST_6R || c == TW_CPST_K || c == TW_CPST_K8; }
+bool k8mode(int c) { return c == TW_CPST_K8 || c == TW_CPST_K8S; }
+bool rmode(int c) { return c == TW_CPST_3R || c == TW_CPST_M3R || c == TW_CPST_6R || c == TW_CPST_K || k8mode(c); }
 bool cmode(int c) { return c == TW_CPST_3C || c == TW_CPST_M3C || c == TW_CPST_6C; }
 
 // ---- host conversions (round to nearest even) ----
@@ -109,7 +110,7 @@ size_t state_size(int precision) { return precision == TW_FP64 ? 8 : (precision 
 // element size of array `which` of a field (0 state, 1 R companion, 2 C
 // companion): the K8 residual holds one byte code per value
 size_t array_size(const tw_config* c, int which) {
-  return which == 1 && c->cpst == TW_CPST_K8 ? 1 : state_size(c->precision);
+  return which == 1 && k8mode(c->cpst) ? 1 : state_size(c->precision);
 }
 size_t coef_size(int precision, int coef16) {
   if (precision == TW_FP64) return 8;
@@ -176,8 +177,8 @@ int check_config(const tw_config* c, std::string* err) {
   for (int a = 0; a < 2; a++)
     if ((c->bc[2 * a] == 'P') != (c->bc[2 * a + 1] == 'P')) return fail(err, TW_EINVAL, "periodic sides must pair");
   if (c->precision < TW_FP64 || c->precision > TW_FP16) return fail(err, TW_EINVAL, "precision");
-  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K8) return fail(err, TW_EINVAL, "cpst");
-  const bool k16 = c->cpst == TW_CPST_K || c->cpst == TW_CPST_K8;
+  if (c->cpst < TW_CPST_0 || c->cpst > TW_CPST_K8S) return fail(err, TW_EINVAL, "cpst");
+  const bool k16 = c->cpst == TW_CPST_K || k8mode(c->cpst);
   if (ref_precision(c->precision) && k16) return fail(err, TW_EINVAL, "cpst K and K8 need 16-bit storage");
   if (!ref_precision(c->precision) && c->cpst != TW_CPST_0 && !k16)
     return fail(err, TW_EINVAL, "16-bit storage supports cpst 0, K or K8");
@@ -1166,7 +1167,7 @@ int tw_get_field_compensated(tw_solver* s, int32_t field, double* out) {
   shape_of(&s->cfg, field_grid(field), &rows, &cols);
   std::vector<double> comp((size_t)rows * cols);
   if ((e = tw_get_companion(s, field, comp.data()))) return e;
-  if (s->cfg.cpst == TW_CPST_K8) {
+  if (k8mode(s->cfg.cpst)) {
     // code q -> q * 2^(max(e, 1) - bias), e the stored value's exponent field
     // (BF16: bias 142, exponent bias 127; FP16: bias 33, exponent bias 15)
     const int bias = s->cfg.precision == TW_BF16 ? 142 : 33, ebias = s->cfg.precision == TW_BF16 ? 127 : 15;

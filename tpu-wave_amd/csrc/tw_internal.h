@@ -83,7 +83,7 @@ inline int64_t pitch_of(int cols, int pad) { return ((int64_t)cols + 127) / 128 
 // 16-bit without a residual at 16.
 inline int32_t default_chunk_rows(int precision, int cpst) {
   const bool sixteen = precision == TW_BF16 || precision == TW_FP16;
-  if (sixteen) return (cpst == TW_CPST_K || cpst == TW_CPST_K8) ? 14 : 16;
+  if (sixteen) return (cpst == TW_CPST_K || cpst == TW_CPST_K8 || cpst == TW_CPST_K8S) ? 14 : 16;
   return precision == TW_FP32 ? 20 : 16;
 }
 

@@ -459,8 +459,8 @@ int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* en
 int tw_result_folder(const tw_input* in, const tw_config* cfg, const char* root, char* out, int32_t cap) {
   if (!in || !cfg || !root || !out || cap <= 0) return err(TW_EINVAL, "bad argument");
   static const char* prec[4] = {"fp64", "fp32", "bf16", "fp16"};
-  static const char* cp[9] = {"0_0", "3_R", "-3_R", "6_R", "3_C", "-3_C", "6_C", "K", "K8"};
-  if (cfg->precision < 0 || cfg->precision > 3 || cfg->cpst < 0 || cfg->cpst > TW_CPST_K8)
+  static const char* cp[10] = {"0_0", "3_R", "-3_R", "6_R", "3_C", "-3_C", "6_C", "K", "K8", "K8S"};
+  if (cfg->precision < 0 || cfg->precision > 3 || cfg->cpst < 0 || cfg->cpst > TW_CPST_K8S)
     return err(TW_EINVAL, "bad configuration");
   char dt[64];
   snprintf(dt, sizeof dt, "dt_%16.15e", cfg->dt);

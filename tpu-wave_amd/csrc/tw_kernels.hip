@@ -150,9 +150,11 @@ __device__ __forceinline__ ST k8_enc(float x, ST hi) {
   const float c = __builtin_fminf(__builtin_fmaxf(q, -127.f), 127.f);
   return __builtin_bit_cast(ST, (int16_t)(int)c);
 }
+// the 8-bit residual variants (K8, and K8S whose stencils read the residual too)
+__host__ __device__ __forceinline__ bool is_k8(int cpst) { return cpst == TW_CPST_K8 || cpst == TW_CPST_K8S; }
 // residual mode of a compensation variant: 1 the 16-bit residual, 2 the byte code
 __host__ __device__ __forceinline__ int kmode_of(int cpst) {
-  return cpst == TW_CPST_K ? 1 : (cpst == TW_CPST_K8 ? 2 : 0);
+  return cpst == TW_CPST_K ? 1 : (is_k8(cpst) ? 2 : 0);
 }
 // companion element q of a per-point access; byte codes widen to the register form
 template <class ST>
@@ -160,6 +162,15 @@ __device__ __forceinline__ ST rd_res(const ST* r, int64_t q, int km) {
   if constexpr (sizeof(ST) == 2)
     if (km == 2) return __builtin_bit_cast(ST, (int16_t)((const int8_t*)r)[q]);
   return r[q];
+}
+// a value as the stencils, SAT terms and boundary values read it: the stored
+// value, or (rf, K8S) the stored value plus its decoded residual in FP32
+template <class ST, class CT>
+__device__ __forceinline__ CT pval(const ST* s, const ST* r, int64_t q, int rf) {
+  CT v = (CT)St<ST>::ld(s[q]);
+  if constexpr (sizeof(ST) == 2)
+    if (rf) v = v + (CT)k8_dec(rd_res(r, q, 2), s[q]);
+  return v;
 }
 template <class ST>
 __device__ __forceinline__ void wr_res(ST* r, int64_t q, ST v, int km) {
@@ -307,8 +318,7 @@ static Args<ST, KT, CT> make_args(const Plan& p, const StepCtl* c, double* part)
 // (1 = N, 0 = M), from partner P.  v = 0; v = v + S*c per tap in tap order.
 template <class ST, class CT>
 __device__ __forceinline__ CT ddx(const F<ST>& P, int out_t, int ix, int iy, int n, int periodic, int M,
-                                  const TabT<CT>& T) {
-  using S = St<ST>;
+                                  const TabT<CT>& T, int rf = 0) {
   CT v = (CT)0;
   if (periodic) {
     int r = ix + (out_t ? -2 : -1);
@@ -316,32 +326,31 @@ __device__ __forceinline__ CT ddx(const F<ST>& P, int out_t, int ix, int iy, int
     for (int k = 0; k < 4; k++) {
       int rr = r + k;
       rr = rr < 0 ? rr + M : (rr >= M ? rr - M : rr);
-      v = v + (CT)S::ld(P.s[P.at(rr, iy)]) * T.st[k];
+      v = v + pval<ST, CT>(P.s, P.r, P.at(rr, iy), rf) * T.st[k];
     }
     return v;
   }
   const int nb = out_t ? 4 : 3;
   if (ix < nb) {
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + (CT)S::ld(P.s[P.at(w, iy)]) * (out_t ? T.DN_L[ix][w] : T.DM_L[ix][w]);
+    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, P.at(w, iy), rf) * (out_t ? T.DN_L[ix][w] : T.DM_L[ix][w]);
   } else if (ix >= n - nb) {
     const int r = ix - (n - nb);
     const int b = P.rows - 5;
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + (CT)S::ld(P.s[P.at(b + w, iy)]) * (out_t ? T.DN_R[r][w] : T.DM_R[r][w]);
+    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, P.at(b + w, iy), rf) * (out_t ? T.DN_R[r][w] : T.DM_R[r][w]);
   } else {
     int r = ix + (out_t ? -2 : -1);
 #pragma unroll
-    for (int k = 0; k < 4; k++) v = v + (CT)S::ld(P.s[P.at(r + k, iy)]) * T.st[k];
+    for (int k = 0; k < 4; k++) v = v + pval<ST, CT>(P.s, P.r, P.at(r + k, iy), rf) * T.st[k];
   }
   return v;
 }
 
 template <class ST, class CT>
 __device__ __forceinline__ CT ddy(const F<ST>& P, int out_t, int ix, int iy, int n, int periodic, int M,
-                                  const TabT<CT>& T) {
-  using S = St<ST>;
-  const ST* row = P.s + (int64_t)ix * P.pitch;
+                                  const TabT<CT>& T, int rf = 0) {
+  const int64_t row = (int64_t)ix * P.pitch;
   CT v = (CT)0;
   if (periodic) {
     int c = iy + (out_t ? -2 : -1);
@@ -349,23 +358,23 @@ __device__ __forceinline__ CT ddy(const F<ST>& P, int out_t, int ix, int iy, int
     for (int k = 0; k < 4; k++) {
       int cc = c + k;
       cc = cc < 0 ? cc + M : (cc >= M ? cc - M : cc);
-      v = v + (CT)S::ld(row[cc]) * T.st[k];
+      v = v + pval<ST, CT>(P.s, P.r, row + cc, rf) * T.st[k];
     }
     return v;
   }
   const int nb = out_t ? 4 : 3;
   if (iy < nb) {
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + (CT)S::ld(row[w]) * (out_t ? T.DN_L[iy][w] : T.DM_L[iy][w]);
+    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, row + w, rf) * (out_t ? T.DN_L[iy][w] : T.DM_L[iy][w]);
   } else if (iy >= n - nb) {
     const int r = iy - (n - nb);
     const int b = P.cols - 5;
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + (CT)S::ld(row[b + w]) * (out_t ? T.DN_R[r][w] : T.DM_R[r][w]);
+    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, row + b + w, rf) * (out_t ? T.DN_R[r][w] : T.DM_R[r][w]);
   } else {
     int c = iy + (out_t ? -2 : -1);
 #pragma unroll
-    for (int k = 0; k < 4; k++) v = v + (CT)S::ld(row[c + k]) * T.st[k];
+    for (int k = 0; k < 4; k++) v = v + pval<ST, CT>(P.s, P.r, row + c + k, rf) * T.st[k];
   }
   return v;
 }
@@ -378,7 +387,6 @@ __device__ __forceinline__ CT ddy(const F<ST>& P, int out_t, int ix, int iy, int
 template <class ST, class KT, class CT>
 __device__ __forceinline__ CT sat_add(const Stat<ST, KT, CT>& A, CT acc, int vf, int pf, int axis, int out_t, int ix,
                                       int iy, CT own, CT coef) {
-  using S = St<ST>;
   const F<ST>& P = A.f[pf];
   const F<ST>& O = A.f[vf];
   const int a = axis == 0 ? ix : iy;          // index along the axis
@@ -403,7 +411,7 @@ __device__ __forceinline__ CT sat_add(const Stat<ST, KT, CT>& A, CT acc, int vf,
     for (int q = 0; q < plen; q++) {
       const int pa = side == 0 ? q : np - plen + q;
       const CT pw = out_t ? (side == 0 ? A.t.projM_L[q] : A.t.projM_R[q]) : (CT)1;
-      const CT sv = axis == 0 ? (CT)S::ld(P.s[P.at(pa, j)]) : (CT)S::ld(P.s[P.at(j, pa)]);
+      const CT sv = pval<ST, CT>(P.s, P.r, axis == 0 ? P.at(pa, j) : P.at(j, pa), A.cpst == TW_CPST_K8S);
       send = send + pw * sv;
     }
     CT recv = (CT)0;
@@ -427,7 +435,7 @@ __device__ __forceinline__ CT acc_init(const F<ST>& X, int64_t q, int cpst) {
   if (cpst == TW_CPST_3R || cpst == TW_CPST_M3R || cpst == TW_CPST_6R || cpst == TW_CPST_K)
     return (CT)St<ST>::ld(X.r[q]);
   if constexpr (sizeof(ST) == 2)
-    if (cpst == TW_CPST_K8) return (CT)k8_dec(rd_res(X.r, q, 2), X.s[q]);
+    if (is_k8(cpst)) return (CT)k8_dec(rd_res(X.r, q, 2), X.s[q]);
   return (CT)0;
 }
 
@@ -497,7 +505,7 @@ __device__ __forceinline__ void update(const F<ST>& X, int64_t q, CT acc, int cp
     const ST hi = S::st(t);
     *snew = hi;
     if (cpst == TW_CPST_K) *rnew = S::st(t - S::ld(hi));
-    if (cpst == TW_CPST_K8) *rnew = k8_enc(t - S::ld(hi), hi);
+    if (is_k8(cpst)) *rnew = k8_enc(t - S::ld(hi), hi);
   }
 }
 
@@ -643,6 +651,7 @@ template <class ST, class KT, class CT>
 __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn& D, int ix, int iy, bool inr) {
   const int el = A.physics == TW_ELASTIC;
   const int kmode = kmode_of(A.cpst);
+  const int rf = A.cpst == TW_CPST_K8S;  // stencils read value + residual
   double e = 0.0;
   // ---- vx on NM: d/dx of sxx (elastic) or p, + d/dy of sxy (elastic)
   {
@@ -650,14 +659,14 @@ __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
     if (inr && ix < X.rows && iy < X.cols) {
       const int64_t q = X.at(ix, iy);
       const CT b = kld<KT, CT>(A.k[TW_C_BINV_VX].get(ix, iy));
-      const CT own = (CT)St<ST>::ld(X.s[q]);
+      const CT own = pval<ST, CT>(X.s, X.r, q, rf);
       CT acc = acc_init<ST, CT>(X, q, A.cpst);
       const int pxf = el ? TW_SXX : TW_P;
-      CT d = ddx<ST, CT>(A.f[pxf], 1, ix, iy, X.rows, A.perx, A.mx, A.t);
+      CT d = ddx<ST, CT>(A.f[pxf], 1, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
       acc = acc + d * b;
       if (!A.perx) acc = sat_add(A, acc, TW_VX, pxf, 0, 1, ix, iy, own, b);
       if (el) {
-        d = ddy<ST, CT>(A.f[TW_SXY], 0, ix, iy, X.cols, A.pery, A.my, A.t);
+        d = ddy<ST, CT>(A.f[TW_SXY], 0, ix, iy, X.cols, A.pery, A.my, A.t, rf);
         acc = acc + d * b;
         if (!A.pery) acc = sat_add(A, acc, TW_VX, TW_SXY, 1, 0, ix, iy, own, b);
       }
@@ -677,15 +686,15 @@ __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
     if (inr && ix < X.rows && iy < X.cols) {
       const int64_t q = X.at(ix, iy);
       const CT b = kld<KT, CT>(A.k[TW_C_BINV_VY].get(ix, iy));
-      const CT own = (CT)St<ST>::ld(X.s[q]);
+      const CT own = pval<ST, CT>(X.s, X.r, q, rf);
       CT acc = acc_init<ST, CT>(X, q, A.cpst);
       if (el) {
-        CT d = ddx<ST, CT>(A.f[TW_SXY], 0, ix, iy, X.rows, A.perx, A.mx, A.t);
+        CT d = ddx<ST, CT>(A.f[TW_SXY], 0, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
         acc = acc + d * b;
         if (!A.perx) acc = sat_add(A, acc, TW_VY, TW_SXY, 0, 0, ix, iy, own, b);
       }
       const int pyf = el ? TW_SYY : TW_P;
-      CT d = ddy<ST, CT>(A.f[pyf], 1, ix, iy, X.cols, A.pery, A.my, A.t);
+      CT d = ddy<ST, CT>(A.f[pyf], 1, ix, iy, X.cols, A.pery, A.my, A.t, rf);
       acc = acc + d * b;
       if (!A.pery) acc = sat_add(A, acc, TW_VY, pyf, 1, 1, ix, iy, own, b);
       ST sn = X.s[q], rn = sn;
@@ -725,6 +734,7 @@ __global__ void __launch_bounds__(256) k_velocity(Args<ST, KT, CT> A, Rects R) {
 template <class ST, class KT, class CT>
 __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn& D, int ix, int iy, bool inr) {
   const int kmode = kmode_of(A.cpst);
+  const int rf = A.cpst == TW_CPST_K8S;  // stencils read value + residual
   const int rmode = kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R;
   double e = 0.0;
   if (A.physics == TW_ELASTIC) {
@@ -734,9 +744,9 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         const int64_t q = X.at(ix, iy);
         const CT mu = kld<KT, CT>(A.k[TW_C_MU].get(ix, iy));
         CT acc = acc_init<ST, CT>(X, q, A.cpst);
-        CT d = ddx<ST, CT>(A.f[TW_VY], 1, ix, iy, X.rows, A.perx, A.mx, A.t);
+        CT d = ddx<ST, CT>(A.f[TW_VY], 1, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
         acc = acc + d * mu;
-        d = ddy<ST, CT>(A.f[TW_VX], 1, ix, iy, X.cols, A.pery, A.my, A.t);
+        d = ddy<ST, CT>(A.f[TW_VX], 1, ix, iy, X.cols, A.pery, A.my, A.t, rf);
         acc = acc + d * mu;
         ST sn = X.s[q], rn = sn;
         if (rmode) rn = rd_res(X.r, q, kmode);
@@ -757,11 +767,11 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         const CT tmu = kld<KT, CT>(A.k[TW_C_TWOMU].get(ix, iy));
         CT a0 = acc_init<ST, CT>(X0, q, A.cpst);
         CT a1 = acc_init<ST, CT>(X1, q, A.cpst);
-        CT v = ddx<ST, CT>(A.f[TW_VX], 0, ix, iy, X0.rows, A.perx, A.mx, A.t);
+        CT v = ddx<ST, CT>(A.f[TW_VX], 0, ix, iy, X0.rows, A.perx, A.mx, A.t, rf);
         a0 = a0 + v * lam;
         a1 = a1 + v * lam;
         a0 = a0 + v * tmu;
-        CT w = ddy<ST, CT>(A.f[TW_VY], 0, ix, iy, X0.cols, A.pery, A.my, A.t);
+        CT w = ddy<ST, CT>(A.f[TW_VY], 0, ix, iy, X0.cols, A.pery, A.my, A.t, rf);
         a0 = a0 + w * lam;
         a1 = a1 + w * lam;
         a1 = a1 + w * tmu;
@@ -801,9 +811,9 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
       const int64_t q = X.at(ix, iy);
       const CT kap = kld<KT, CT>(A.k[TW_C_KAPPA].get(ix, iy));
       CT acc = acc_init<ST, CT>(X, q, A.cpst);
-      CT d = ddx<ST, CT>(A.f[TW_VX], 0, ix, iy, X.rows, A.perx, A.mx, A.t);
+      CT d = ddx<ST, CT>(A.f[TW_VX], 0, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
       acc = acc + d * kap;
-      d = ddy<ST, CT>(A.f[TW_VY], 0, ix, iy, X.cols, A.pery, A.my, A.t);
+      d = ddy<ST, CT>(A.f[TW_VY], 0, ix, iy, X.cols, A.pery, A.my, A.t, rf);
       acc = acc + d * kap;
       ST sn = X.s[q], rn = sn;
       if (rmode) rn = rd_res(X.r, q, kmode);
@@ -926,7 +936,7 @@ __global__ void __launch_bounds__(256) k_stress_energy(Args<ST, KT, CT> A) {
 // vy (MN) on the x sides, vx (NM) on the y sides (projection p_M, namespace_forward.hpp:153-154).
 template <class ST, class KT, class CT>
 __global__ void __launch_bounds__(256) k_uhat(Args<ST, KT, CT> A) {
-  using S = St<ST>;
+  const int rf = A.cpst == TW_CPST_K8S;
   const int j = blockIdx.x * blockDim.x * blockDim.y + threadIdx.y * blockDim.x + threadIdx.x;
   for (int side = 0; side < 4; side++) {
     const int axis = side >> 1, right = side & 1;
@@ -941,7 +951,7 @@ __global__ void __launch_bounds__(256) k_uhat(Args<ST, KT, CT> A) {
     CT u = (CT)0;
     for (int w = 0; w < 3; w++) {
       const int a = right ? n_along - 3 + w : w;
-      const CT v = axis == 0 ? (CT)S::ld(X.s[X.at(a, j)]) : (CT)S::ld(X.s[X.at(j, a)]);
+      const CT v = pval<ST, CT>(X.s, X.r, axis == 0 ? X.at(a, j) : X.at(j, a), rf);
       u = u + pm[w] * v;
     }
     out[j] = u;
@@ -1096,7 +1106,26 @@ __device__ __forceinline__ CT ldcol(const ST* rowp, int v, int periodic, int M, 
 
 // fill the LDS segment of partner row `row` for virtual columns base-HALO .. base+STRIP+HALO
 template <class ST, class CT>
-__device__ __forceinline__ void seg_fill(CT* seg, const F<ST>& P, int row, int base, int lane, int periodic, int M) {
+__device__ __forceinline__ void seg_fill(CT* seg, const F<ST>& P, int row, int base, int lane, int periodic, int M,
+                                         int rf = 0) {
+  if (rf) {  // K8S: value + residual, element by element (edge strips only)
+    const int64_t r0 = (int64_t)row * P.pitch;
+    auto col = [&](int v) -> CT {
+      if (periodic) v = ((v % M) + M) % M;
+      if (v < 0 || v >= P.cols) return (CT)0;
+      return pval<ST, CT>(P.s, P.r, r0 + v, 1);
+    };
+#pragma unroll
+    for (int j = 0; j < 4; j++) seg[HALO + lane * VW + j] = col(base + lane * VW + j);
+    if (lane < 2) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) seg[lane * 4 + j] = col(base - HALO + lane * 4 + j);
+    } else if (lane >= 62) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) seg[HALO + STRIP + (lane - 62) * 4 + j] = col(base + STRIP + (lane - 62) * 4 + j);
+    }
+    return;
+  }
   const ST* rowp = P.s + (int64_t)row * P.pitch;
   const int v0 = base + lane * VW;
   CT o[4];
@@ -1181,7 +1210,33 @@ struct FR {
   __device__ __forceinline__ gptr<int8_t> brow(int i) const { return (gptr<int8_t>)r + (int64_t)i * pitch; }
 };
 
-template <class ST, class CT>
+// K8S reads partner values with their residual: the byte codes of a lane's
+// four values of a row (c a multiple of 4, the same addressing as the values)
+template <class ST>
+__device__ __forceinline__ uint32_t ld_c4(const FR<ST>& P, int row, int c) {
+  typedef __attribute__((address_space(1))) const uint32_t* Q;
+  return *(Q)(P.brow(row) + c);
+}
+// four stored values -> arithmetic type, plus (RF) their decoded residuals
+template <class ST, class CT, bool RF>
+__device__ __forceinline__ void cvt_rf(const V4<ST>& v, uint32_t w, CT* o) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    o[j] = (CT)St<ST>::ld(v.v[j]);
+    if constexpr (RF && sizeof(ST) == 2)
+      o[j] = o[j] + (CT)k8_dec(__builtin_bit_cast(ST, (int16_t)(int8_t)(w >> (8 * j))), v.v[j]);
+  }
+}
+// columns c..c+3 of row `row` (valid) as the stencils read them
+template <class ST, class CT, bool RF>
+__device__ __forceinline__ void ld4f(const FR<ST>& P, int row, int c, CT* o) {
+  const V4<ST> t = ld_v4<ST>(P.row(row) + c);
+  uint32_t w = 0;
+  if constexpr (RF) w = ld_c4<ST>(P, row, c);
+  cvt_rf<ST, CT, RF>(t, w, o);
+}
+
+template <class ST, class CT, bool RF = false>
 __device__ __forceinline__ void ldx_r(const FR<ST>& P, int row, int c, int periodic, int M, CT o[4]) {
   if (periodic) row = row < 0 ? row + M : (row >= M ? row - M : row);
   if (row < 0 || row >= P.rows || c >= P.cols) {
@@ -1189,10 +1244,10 @@ __device__ __forceinline__ void ldx_r(const FR<ST>& P, int row, int c, int perio
     for (int j = 0; j < 4; j++) o[j] = (CT)0;
     return;
   }
-  ld4<ST, CT>(P.row(row), c, o);
+  ld4f<ST, CT, RF>(P, row, c, o);
 }
 
-template <class ST, class CT>
+template <class ST, class CT, bool RF = false>
 __device__ __forceinline__ void seg_fill_r(CT* seg, const FR<ST>& P, int row, int base, int lane, int periodic, int M) {
   F<ST> f;
   f.s = (ST*)P.s;
@@ -1201,7 +1256,7 @@ __device__ __forceinline__ void seg_fill_r(CT* seg, const FR<ST>& P, int row, in
   f.rows = P.rows;
   f.cols = P.cols;
   f.pitch = P.pitch;
-  seg_fill<ST, CT>(seg, f, row, base, lane, periodic, M);
+  seg_fill<ST, CT>(seg, f, row, base, lane, periodic, M, RF);
 }
 
 // 1/x in FP64 for the energy weights: hardware reciprocal refined by one
@@ -1237,16 +1292,18 @@ __device__ __forceinline__ double energy_core(int eform, double dx, CT v, CT prm
 // and written; km, the residual mode of value_of / add_source (kmode_of)
 template <int CP>
 struct CPT {
+  static constexpr bool k8 = CP == TW_CPST_K8 || CP == TW_CPST_K8S;  // residual as a byte code
   static constexpr bool rm =
-      CP == TW_CPST_3R || CP == TW_CPST_M3R || CP == TW_CPST_6R || CP == TW_CPST_K || CP == TW_CPST_K8;
-  static constexpr int km = CP == TW_CPST_K ? 1 : (CP == TW_CPST_K8 ? 2 : 0);
+      CP == TW_CPST_3R || CP == TW_CPST_M3R || CP == TW_CPST_6R || CP == TW_CPST_K || k8;
+  static constexpr int km = CP == TW_CPST_K ? 1 : (k8 ? 2 : 0);
+  static constexpr bool rf = CP == TW_CPST_K8S;  // partner values read with their residual
 };
 
 // a lane's four residual values of row `row` at column c (multiple of 4):
 // storage-type elements, or four byte codes for K8
 template <class ST, int CP>
 __device__ __forceinline__ V4<ST> ld_r4(const FR<ST>& X, int row, int c) {
-  if constexpr (CP == TW_CPST_K8) {
+  if constexpr (CPT<CP>::k8) {
     typedef __attribute__((address_space(1))) const uint32_t* Q;
     const uint32_t w = *(Q)(X.brow(row) + c);
     V4<ST> o;
@@ -1261,7 +1318,7 @@ __device__ __forceinline__ V4<ST> ld_r4(const FR<ST>& X, int row, int c) {
 // ld_r4 for residual rows read once per half step (non-temporal, as ld_v4s)
 template <class ST, int CP>
 __device__ __forceinline__ V4<ST> ld_r4s(const FR<ST>& X, int row, int c) {
-  if constexpr (CP == TW_CPST_K8) {
+  if constexpr (CPT<CP>::k8) {
     typedef __attribute__((address_space(1))) const uint32_t* Q;
     const uint32_t w = __builtin_nontemporal_load((Q)(X.brow(row) + c));
     V4<ST> o;
@@ -1276,7 +1333,7 @@ __device__ __forceinline__ V4<ST> ld_r4s(const FR<ST>& X, int row, int c) {
 // store of a lane's four residual values, element-wise where only some columns are in
 template <class ST, int CP>
 __device__ __forceinline__ void st_r4(const FR<ST>& X, int row, int c, const V4<ST>& v, bool full4, const bool* in) {
-  if constexpr (CP == TW_CPST_K8) {
+  if constexpr (CPT<CP>::k8) {
     const auto p = X.brow(row) + c;
     if (full4) {
       uint32_t w = 0;
@@ -1303,7 +1360,7 @@ __device__ __forceinline__ void st_r4(const FR<ST>& X, int row, int c, const V4<
 // accumulator start of a compensated update: R or the decoded residual, else zero
 template <class ST, class CT, int CP>
 __device__ __forceinline__ CT acc0(ST r, ST s) {
-  if constexpr (CP == TW_CPST_K8)
+  if constexpr (CPT<CP>::k8)
     return (CT)k8_dec(r, s);
   else if constexpr (CPT<CP>::rm)
     return (CT)St<ST>::ld(r);
@@ -1360,7 +1417,7 @@ __device__ __forceinline__ void upd(ST& s, ST& r, CT acc) {
     const ST hi = S::st(t);
     s = hi;
     if constexpr (CP == TW_CPST_K) r = S::st(t - S::ld(hi));
-    if constexpr (CP == TW_CPST_K8) r = k8_enc(t - S::ld(hi), hi);
+    if constexpr (CPT<CP>::k8) r = k8_enc(t - S::ld(hi), hi);
   }
 }
 
@@ -1426,25 +1483,24 @@ __device__ __forceinline__ double energy_mm_core(double dx, CT lam, CT tmu, CT s
 }
 
 // segment fill for a strip whose halo lies inside the row (no wrap, no bounds)
-template <class ST, class CT>
+template <class ST, class CT, bool RF = false>
 __device__ __forceinline__ void seg_fill_fast(CT* seg, const FR<ST>& P, int row, int base, int lane) {
-  const auto rp = P.row(row);
   CT o[4];
-  ld4<ST, CT>(rp, base + lane * VW, o);
+  ld4f<ST, CT, RF>(P, row, base + lane * VW, o);
 #pragma unroll
   for (int j = 0; j < 4; j++) seg[HALO + lane * VW + j] = o[j];
   if (lane < 2) {
-    ld4<ST, CT>(rp, base - HALO + lane * 4, o);
+    ld4f<ST, CT, RF>(P, row, base - HALO + lane * 4, o);
 #pragma unroll
     for (int j = 0; j < 4; j++) seg[lane * 4 + j] = o[j];
   } else if (lane >= 62) {
-    ld4<ST, CT>(rp, base + STRIP + (lane - 62) * 4, o);
+    ld4f<ST, CT, RF>(P, row, base + STRIP + (lane - 62) * 4, o);
 #pragma unroll
     for (int j = 0; j < 4; j++) seg[HALO + STRIP + (lane - 62) * 4 + j] = o[j];
   }
 }
 
-template <bool FULL, class ST, class CT>
+template <bool FULL, class ST, class CT, bool RF = false>
 __device__ __forceinline__ void ldx_f(const FR<ST>& P, int row, int c, int periodic, int M, CT o[4]) {
   if constexpr (FULL) {
     if (periodic) row = row < 0 ? row + M : (row >= M ? row - M : row);
@@ -1453,18 +1509,18 @@ __device__ __forceinline__ void ldx_f(const FR<ST>& P, int row, int c, int perio
       for (int j = 0; j < 4; j++) o[j] = (CT)0;
       return;
     }
-    ld4<ST, CT>(P.row(row), c, o);
+    ld4f<ST, CT, RF>(P, row, c, o);
   } else {
-    ldx_r<ST, CT>(P, row, c, periodic, M, o);
+    ldx_r<ST, CT, RF>(P, row, c, periodic, M, o);
   }
 }
 
-template <bool FULL, class ST, class CT>
+template <bool FULL, class ST, class CT, bool RF = false>
 __device__ __forceinline__ void seg_f(CT* seg, const FR<ST>& P, int row, int base, int lane, int periodic, int M) {
   if constexpr (FULL)
-    seg_fill_fast<ST, CT>(seg, P, row, base, lane);
+    seg_fill_fast<ST, CT, RF>(seg, P, row, base, lane);
   else
-    seg_fill_r<ST, CT>(seg, P, row, base, lane, periodic, M);
+    seg_fill_r<ST, CT, RF>(seg, P, row, base, lane, periodic, M);
 }
 
 template <bool FULL, class ST, int CP>
@@ -1548,6 +1604,13 @@ template <class ST, class CT>
 __device__ __forceinline__ void seg_put4(CT* p, const V4<ST>& v) {
 #pragma unroll
   for (int j = 0; j < 4; j++) p[j] = (CT)St<ST>::ld(v.v[j]);
+}
+template <class ST, class CT, bool RF>
+__device__ __forceinline__ void seg_put4r(CT* p, const V4<ST>& v, uint32_t w) {
+  CT o[4];
+  cvt_rf<ST, CT, RF>(v, w, o);
+#pragma unroll
+  for (int j = 0; j < 4; j++) p[j] = o[j];
 }
 
 // core-column mask of a lane's four columns; true when all four are in
@@ -1634,6 +1697,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
                                           int by, int nbx) {
   constexpr bool RM = CPT<CP>::rm;
   constexpr int KM = CPT<CP>::km;
+  constexpr bool RF = CPT<CP>::rf;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // sxy row (y partner of vx, elastic)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // syy or p row (y partner of vy)
   __shared__ double sh[8];
@@ -1682,22 +1746,22 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
     const bool anyy = iny[0] || iny[1] || iny[2] || iny[3];
     CT wa[4][4], wb[4][4];  // vx: x partner rows r-2..r+1 ; vy: sxy rows r-1..r+2
 #pragma unroll
-    for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(PX, x0 - 2 + k, c, perx, mx, wa[k + 1]);
+    for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT, RF>(PX, x0 - 2 + k, c, perx, mx, wa[k + 1]);
     if constexpr (PH == TW_ELASTIC) {
 #pragma unroll
-      for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(SXY, x0 - 1 + k, c, perx, mx, wb[k + 1]);
+      for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT, RF>(SXY, x0 - 1 + k, c, perx, mx, wb[k + 1]);
     }
     int buf = 0;
     for (int r = x0; r < x1; r++) {
       shift_win(wa);
-      ldx_f<FULL, ST, CT>(PX, r + 1, c, perx, mx, wa[3]);
+      ldx_f<FULL, ST, CT, RF>(PX, r + 1, c, perx, mx, wa[3]);
       if constexpr (PH == TW_ELASTIC) {
         shift_win(wb);
-        ldx_f<FULL, ST, CT>(SXY, r + 2, c, perx, mx, wb[3]);
-        seg_f<FULL, ST, CT>(segA[buf], SXY, r, base, lane, pery, my);
+        ldx_f<FULL, ST, CT, RF>(SXY, r + 2, c, perx, mx, wb[3]);
+        seg_f<FULL, ST, CT, RF>(segA[buf], SXY, r, base, lane, pery, my);
       }
       const bool vy_row = r < VY.rows;
-      if (vy_row) seg_f<FULL, ST, CT>(segB[buf], PY, r, base, lane, pery, my);
+      if (vy_row) seg_f<FULL, ST, CT, RF>(segB[buf], PY, r, base, lane, pery, my);
       __syncthreads();
       if (anyx && r < VX.rows) {  // ---- vx (NM)
         CT b[4];
@@ -1765,13 +1829,13 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
     const int cl = min(c, cmax);  // load column: inside the allocation for every lane
     CT A0[4], A1[4], A2[4], A3[4];  // PX rows r-2..r+1 (x partner of vx)
     CT B0[4], B1[4], B2[4], B3[4];  // SXY rows r-1..r+2 (x partner of vy), elastic
-    ldx_f<true, ST, CT>(PX, x0 - 2, cl, perx, mx, A0);
-    ldx_f<true, ST, CT>(PX, x0 - 1, cl, perx, mx, A1);
-    ldx_f<true, ST, CT>(PX, x0, cl, perx, mx, A2);
+    ldx_f<true, ST, CT, RF>(PX, x0 - 2, cl, perx, mx, A0);
+    ldx_f<true, ST, CT, RF>(PX, x0 - 1, cl, perx, mx, A1);
+    ldx_f<true, ST, CT, RF>(PX, x0, cl, perx, mx, A2);
     if constexpr (PH == TW_ELASTIC) {
-      ldx_f<true, ST, CT>(SXY, x0 - 1, cl, perx, mx, B0);
-      ldx_f<true, ST, CT>(SXY, x0, cl, perx, mx, B1);
-      ldx_f<true, ST, CT>(SXY, x0 + 1, cl, perx, mx, B2);
+      ldx_f<true, ST, CT, RF>(SXY, x0 - 1, cl, perx, mx, B0);
+      ldx_f<true, ST, CT, RF>(SXY, x0, cl, perx, mx, B1);
+      ldx_f<true, ST, CT, RF>(SXY, x0 + 1, cl, perx, mx, B2);
     }
     const int hc = halo_col(lane, base, cl, cmax);
     const bool hl = lane < 2 || lane >= 62;
@@ -1790,6 +1854,16 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         oB = ld_v4<ST>(PY.row(ry) + cl);   // syy row r
       }
       const V4<ST> hB = ld_v4<ST>(PY.row(ry) + hc);  // syy / p row r halo
+      uint32_t cnA = 0, cnB = 0, chA = 0, coB = 0, chB = 0;  // their residual codes (K8S)
+      if constexpr (RF) {
+        cnA = ld_c4<ST>(PX, wrap_row(r + 1, perx, mx), cl);
+        if constexpr (PH == TW_ELASTIC) {
+          cnB = ld_c4<ST>(SXY, wrap_row(r + 2, perx, mx), cl);
+          chA = ld_c4<ST>(SXY, r, hc);
+          coB = ld_c4<ST>(PY, ry, cl);
+        }
+        chB = ld_c4<ST>(PY, ry, hc);
+      }
       const Q4<KT> kx = kq4<KT>(KBX, rx, cl);
       V4<ST> sx = ld_v4s<ST>(VX.row(rx) + cl);
       V4<ST> qx = sx;
@@ -1799,24 +1873,24 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       V4<ST> qy = sy;
       if constexpr (RM) qy = ld_r4s<ST, CP>(VY, ry, cl);
       // ---- segments
-      cvt_v4<ST, CT>(nA, a3);
+      cvt_rf<ST, CT, RF>(nA, cnA, a3);
       if constexpr (PH == TW_ELASTIC) {
-        cvt_v4<ST, CT>(nB, b3);
+        cvt_rf<ST, CT, RF>(nB, cnB, b3);
         if (perx && r >= mx) {  // duplicated node row: not the window's row
           CT o[4];
-          ld4<ST, CT>(SXY.row(r), cl, o);
+          ld4f<ST, CT, RF>(SXY, r, cl, o);
           seg_own<CT>(segA[buf], o, lane);
         } else {
           seg_own<CT>(segA[buf], b1, lane);
         }
-        if (hl) seg_put4<ST, CT>(segA[buf] + hs, hA);
+        if (hl) seg_put4r<ST, CT, RF>(segA[buf] + hs, hA, chA);
         CT o[4];
-        cvt_v4<ST, CT>(oB, o);
+        cvt_rf<ST, CT, RF>(oB, coB, o);
         seg_own<CT>(segB[buf], o, lane);
       } else {
         seg_own<CT>(segB[buf], a2, lane);  // p row r
       }
-      if (hl) seg_put4<ST, CT>(segB[buf] + hs, hB);
+      if (hl) seg_put4r<ST, CT, RF>(segB[buf] + hs, hB, chB);
       __syncthreads();
       if (vx_row) {  // ---- vx (NM)
         CT b[4];
@@ -1900,6 +1974,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
                                           int by, int nbx) {
   constexpr bool RM = CPT<CP>::rm;
   constexpr int KM = CPT<CP>::km;
+  constexpr bool RF = CPT<CP>::rf;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // vx row (y partner of sxy)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // vy row (y partner of MM / p)
   __shared__ double sh[8];
@@ -1952,21 +2027,21 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
     CT wa[4][4], wb[4][4];  // sxy: vy rows r-2..r+1 ; MM: vx rows r-1..r+2
     if constexpr (PH == TW_ELASTIC) {
 #pragma unroll
-      for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(VY, x0 - 2 + k, c, perx, mx, wa[k + 1]);
+      for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT, RF>(VY, x0 - 2 + k, c, perx, mx, wa[k + 1]);
     }
 #pragma unroll
-    for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT>(VX, x0 - 1 + k, c, perx, mx, wb[k + 1]);
+    for (int k = 0; k < 3; k++) ldx_f<FULL, ST, CT, RF>(VX, x0 - 1 + k, c, perx, mx, wb[k + 1]);
     int buf = 0;
     for (int r = x0; r < x1; r++) {
       if constexpr (PH == TW_ELASTIC) {
         shift_win(wa);
-        ldx_f<FULL, ST, CT>(VY, r + 1, c, perx, mx, wa[3]);
-        seg_f<FULL, ST, CT>(segA[buf], VX, r, base, lane, pery, my);
+        ldx_f<FULL, ST, CT, RF>(VY, r + 1, c, perx, mx, wa[3]);
+        seg_f<FULL, ST, CT, RF>(segA[buf], VX, r, base, lane, pery, my);
       }
       shift_win(wb);
-      ldx_f<FULL, ST, CT>(VX, r + 2, c, perx, mx, wb[3]);
+      ldx_f<FULL, ST, CT, RF>(VX, r + 2, c, perx, mx, wb[3]);
       const bool mm_row = r < X0.rows;
-      if (mm_row) seg_f<FULL, ST, CT>(segB[buf], VY, r, base, lane, pery, my);
+      if (mm_row) seg_f<FULL, ST, CT, RF>(segB[buf], VY, r, base, lane, pery, my);
       __syncthreads();
       if constexpr (PH == TW_ELASTIC) {
         if (anyn && r < XN.rows) {  // ---- sxy (NN)
@@ -2075,13 +2150,13 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
     CT A0[4], A1[4], A2[4], A3[4];  // VY rows r-2..r+1 (x partner of sxy), elastic
     CT B0[4], B1[4], B2[4], B3[4];  // VX rows r-1..r+2 (x partner of MM)
     if constexpr (PH == TW_ELASTIC) {
-      ldx_f<true, ST, CT>(VY, x0 - 2, cl, perx, mx, A0);
-      ldx_f<true, ST, CT>(VY, x0 - 1, cl, perx, mx, A1);
-      ldx_f<true, ST, CT>(VY, x0, cl, perx, mx, A2);
+      ldx_f<true, ST, CT, RF>(VY, x0 - 2, cl, perx, mx, A0);
+      ldx_f<true, ST, CT, RF>(VY, x0 - 1, cl, perx, mx, A1);
+      ldx_f<true, ST, CT, RF>(VY, x0, cl, perx, mx, A2);
     }
-    ldx_f<true, ST, CT>(VX, x0 - 1, cl, perx, mx, B0);
-    ldx_f<true, ST, CT>(VX, x0, cl, perx, mx, B1);
-    ldx_f<true, ST, CT>(VX, x0 + 1, cl, perx, mx, B2);
+    ldx_f<true, ST, CT, RF>(VX, x0 - 1, cl, perx, mx, B0);
+    ldx_f<true, ST, CT, RF>(VX, x0, cl, perx, mx, B1);
+    ldx_f<true, ST, CT, RF>(VX, x0 + 1, cl, perx, mx, B2);
     const int hc = halo_col(lane, base, cl, cmax);
     const bool hl = lane < 2 || lane >= 62;
     const int hs = lane < 2 ? lane * 4 : HALO + STRIP + (lane - 62) * 4;
@@ -2105,6 +2180,17 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       }
       const V4<ST> nB = ld_v4<ST>(VX.row(wrap_row(r + 2, perx, mx)) + cl);
       const V4<ST> hB = ld_v4<ST>(VY.row(rm) + hc);  // vy row r halo
+      uint32_t cnA = 0, cnB = 0, chA = 0, coB = 0, chB = 0;  // their residual codes (K8S)
+      if constexpr (RF) {
+        if constexpr (PH == TW_ELASTIC) {
+          cnA = ld_c4<ST>(VY, wrap_row(r + 1, perx, mx), cl);
+          chA = ld_c4<ST>(VX, r, hc);
+        } else {
+          coB = ld_c4<ST>(VY, rm, cl);
+        }
+        cnB = ld_c4<ST>(VX, wrap_row(r + 2, perx, mx), cl);
+        chB = ld_c4<ST>(VY, rm, hc);
+      }
       const Q4<KT> k0 = kq4<KT>(K0, rm, cl);
       Q4<KT> k1;
       if constexpr (PH == TW_ELASTIC) k1 = kq4<KT>(K1, rm, cl);
@@ -2117,24 +2203,24 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         if constexpr (RM) q1 = ld_r4s<ST, CP>(X1, rm, cl);
       }
       // ---- segments
-      cvt_v4<ST, CT>(nB, b3);
+      cvt_rf<ST, CT, RF>(nB, cnB, b3);
       if constexpr (PH == TW_ELASTIC) {
-        cvt_v4<ST, CT>(nA, a3);
+        cvt_rf<ST, CT, RF>(nA, cnA, a3);
         if (perx && r >= mx) {  // duplicated node row of vx: not the window's row
           CT o[4];
-          ld4<ST, CT>(VX.row(r), cl, o);
+          ld4f<ST, CT, RF>(VX, r, cl, o);
           seg_own<CT>(segA[buf], o, lane);
         } else {
           seg_own<CT>(segA[buf], b1, lane);
         }
-        if (hl) seg_put4<ST, CT>(segA[buf] + hs, hA);
+        if (hl) seg_put4r<ST, CT, RF>(segA[buf] + hs, hA, chA);
         seg_own<CT>(segB[buf], a2, lane);  // vy row r
       } else {
         CT o[4];
-        cvt_v4<ST, CT>(oB, o);
+        cvt_rf<ST, CT, RF>(oB, coB, o);
         seg_own<CT>(segB[buf], o, lane);
       }
-      if (hl) seg_put4<ST, CT>(segB[buf] + hs, hB);
+      if (hl) seg_put4r<ST, CT, RF>(segB[buf] + hs, hB, chB);
       __syncthreads();
       if constexpr (PH == TW_ELASTIC) {  // ---- sxy (NN); every core row is an NN row
         CT mu[4];
@@ -2348,6 +2434,8 @@ static hipError_t launch_core_p(const Plan& p, int half, const Stat<ST, KT, CT>*
       launch_core_t<ST, KT, CT, PH, TW_CPST_K>(half, sp, d, C, g, b, s, R, nband);
     else if (p.cpst == TW_CPST_K8)
       launch_core_t<ST, KT, CT, PH, TW_CPST_K8>(half, sp, d, C, g, b, s, R, nband);
+    else if (p.cpst == TW_CPST_K8S)
+      launch_core_t<ST, KT, CT, PH, TW_CPST_K8S>(half, sp, d, C, g, b, s, R, nband);
     else
       launch_core_t<ST, KT, CT, PH, TW_CPST_0>(half, sp, d, C, g, b, s, R, nband);
   }
@@ -2801,7 +2889,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(
 bool fused_ok(const Plan& p) {
   return p.physics == TW_ELASTIC && !p.perx && !p.pery && p.energy_form == 0 && p.mx >= 12 && p.my >= 12 &&
          p.coef16 != 2 &&
-         !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C || p.cpst == TW_CPST_K8);
+         !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C || is_k8(p.cpst));
 }
 
 #endif
@@ -2977,6 +3065,8 @@ static void launch_batch_core(const Plan& p, int half, const BatchCore* items, i
       go(std::integral_constant<int, TW_CPST_K>{});
     else if (p.cpst == TW_CPST_K8)
       go(std::integral_constant<int, TW_CPST_K8>{});
+    else if (p.cpst == TW_CPST_K8S)
+      go(std::integral_constant<int, TW_CPST_K8S>{});
     else
       go(std::integral_constant<int, TW_CPST_0>{});
   }

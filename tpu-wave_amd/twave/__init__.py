@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libtwave.so")
 ELASTIC, ACOUSTIC = 0, 1
 FP64, FP32, BF16, FP16 = 0, 1, 2, 3
 PRECISION = {"fp64": FP64, "fp32": FP32, "bf16": BF16, "fp16": FP16}
-CPST = {"0": 0, "3R": 1, "-3R": 2, "6R": 3, "3C": 4, "-3C": 5, "6C": 6, "K": 7, "K8": 8}
+CPST = {"0": 0, "3R": 1, "-3R": 2, "6R": 3, "3C": 4, "-3C": 5, "6C": 6, "K": 7, "K8": 8, "K8S": 9}
 VX, VY, SXX, SYY, SXY, P = 0, 1, 2, 3, 4, 5
 FIELD_NAMES = {VX: "vx", VY: "vy", SXX: "sxx", SYY: "syy", SXY: "sxy", P: "p"}
 GRID_NN, GRID_MN, GRID_NM, GRID_MM = 0, 1, 2, 3
