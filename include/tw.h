@@ -236,6 +236,13 @@ int tw_set_stream(tw_solver* s, void* hip_stream);
 
 /* Check the state for non-finite values (TW_ENONFINITE if any). */
 int tw_check_finite(tw_solver* s);
+/* tw_check_finite without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The flag lives in a
+ * fixed 256-byte device array per device (its size does not depend on the
+ * grid, so no input exceeds it); it is cleared on every call, and calls that
+ * overlap on one device race for it. */
+int tw_check_finite_noalloc(tw_solver* s);
 
 /* Configuration the solver was created with. */
 int tw_get_config(const tw_solver* s, tw_config* cfg);

@@ -22,6 +22,7 @@ FLAGS = [
 ]
 UNITS = [("csrc/tw_kernels.hip", "tw_kernels_p%d.o" % k, ["-DTW_PREC=%d" % k]) for k in range(4)] + [
     ("csrc/tw_dispatch.cpp", "tw_dispatch.o", []),
+    ("csrc/tw_noalloc.hip", "tw_noalloc.o", []),
     ("csrc/tw_host.cpp", "tw_host.o", []),
     ("csrc/tw_io.cpp", "tw_io.o", []),
 ]

@@ -584,6 +584,15 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     delete s;
     return TW_EDEVICE;
   }
+  // set-up of the allocation-free entry point: its scratch address on this
+  // device (looked up once per device, the device current)
+  int32_t* scratch_ptr = nullptr;
+  he = tw::scratch_address(c->device, &scratch_ptr);
+  if (he != hipSuccess) {
+    hip_fail(s, he, "hipGetSymbolAddress(scratch)");
+    delete s;
+    return TW_EDEVICE;
+  }
   p.physics = c->physics;
   p.precision = c->precision;
   p.coef16 = c->coef16;
@@ -1295,6 +1304,17 @@ int tw_check_finite(tw_solver* s) {
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(flag);
   if (e != hipSuccess) return hip_fail(s, e, "check_finite");
+  if (h) return fail(&s->err, TW_ENONFINITE, "non-finite value in the state");
+  return TW_OK;
+}
+
+int tw_check_finite_noalloc(tw_solver* s) {
+  if (!s) return fail(nullptr, TW_EINVAL, "null handle");
+  (void)hipSetDevice(s->cfg.device);
+  int32_t h = 0;
+  hipError_t e = tw::check_finite_scratch(s->plan, s->cfg.device, s->stream, &h);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return hip_fail(s, e, "check_finite_noalloc");
   if (h) return fail(&s->err, TW_ENONFINITE, "non-finite value in the state");
   return TW_OK;
 }

@@ -232,6 +232,11 @@ hipError_t launch_uhat(const Plan& p, hipStream_t s);
 hipError_t launch_stress_energy(const Plan& p, hipStream_t s);
 hipError_t launch_finalize(const Plan& p, const Pending& pend, hipStream_t s);
 hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s);
+// allocation-free finiteness check (tw_noalloc.hip): device address of the
+// per-device scratch array (resolved once per device, the device current),
+// and the check with its flag there, copied to *host_flag on stream s
+hipError_t scratch_address(int device, int32_t** out);
+hipError_t check_finite_scratch(const Plan& p, int device, hipStream_t s, int32_t* host_flag);
 hipError_t launch_batch_half(const Plan& p, int half, const BatchCore* core, int ncore, int core_blocks,
                              const BatchBand* band, int nband, int band_blocks, hipStream_t s);
 // core and band share of one problem (d, sp and b0 are left to the caller)

@@ -136,6 +136,7 @@ _SIGS = [
     ("tw_checkpoint_load", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     ("tw_set_stream", ctypes.c_int, [_vp, _vp]),
     ("tw_check_finite", ctypes.c_int, [_vp]),
+    ("tw_check_finite_noalloc", ctypes.c_int, [_vp]),
     ("tw_get_config", ctypes.c_int, [_vp, _cp]),
     ("tw_input_default", None, [ctypes.POINTER(Input)]),
     ("tw_read_input_file", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Input)]),
@@ -420,6 +421,11 @@ class Solver:
 
     def check_finite(self):
         e = lib().tw_check_finite(self.h)
+        return e == 0 if e in (0, 5) else self._chk(e)
+
+    def check_finite_noalloc(self):
+        """check_finite without device allocation (tw_check_finite_noalloc)."""
+        e = lib().tw_check_finite_noalloc(self.h)
         return e == 0 if e in (0, 5) else self._chk(e)
 
     def data_misfit(self, data):
