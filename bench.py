@@ -126,6 +126,31 @@ def copy_calibration(torch, nbytes, reps):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps + 1, "copy_read_plus_write_GBps": 2 * nbytes / t / 1e9}))
 
 
+def stream_copy_bandwidth(torch, T, nbytes=1 << 30, reps=20):
+    """Read + write bandwidth (GB/s) of the library's hand-written streaming
+    copy (tw_stream_copy: 16-byte non-temporal loads and stores) of nbytes,
+    timed with HIP events on its stream: the attainable rate of a kernel that
+    moves every byte once, the ceiling the solver's fraction is read against."""
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    st = torch.cuda.Stream()
+    L = T.lib()
+    for _ in range(2):
+        assert L.tw_stream_copy(b.data_ptr(), a.data_ptr(), nbytes, st.cuda_stream) == 0
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        L.tw_stream_copy(b.data_ptr(), a.data_ptr(), nbytes, st.cuda_stream)
+    e1.record(st)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    return 2 * nbytes / t / 1e9
+
+
 def make_solver(T, W, m, precision, cpst, coef16, nt, model_from=None):
     """The elastic workload on a new solver; model_from: take that solver's
     (quantized) coefficients instead of the exact interpolation."""
@@ -424,6 +449,10 @@ def main():
     else:
         roof = dict(step, bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",
                     kernel="one step = velocity + stress half-step kernels")
+    copy_gbps = stream_copy_bandwidth(torch, T)
+    roof["copy_attainable"] = {"GBps": copy_gbps, "frac_of_peak": copy_gbps / HBM_PEAK_GBS,
+                               "kernel": "tw_stream_copy, 1 GiB, 16-byte non-temporal loads and stores"}
+    roof["frac_of_attainable"] = roof["achieved"] / copy_gbps
     out = {
         "metric": METRIC,
         "value": value,

@@ -339,6 +339,14 @@ int tw_setup_shot(tw_solver* s, const tw_input* in, const tw_srcrcv_entry* entri
  * must record exactly this shot's receivers (as tw_setup_shot registers
  * them), otherwise TW_ESTATE. */
 int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* entries, int32_t n, int32_t src_index);
+/* Final-state snapshots as the reference's device driver writes them
+ * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:788-819): for each grid in the
+ * order NN, MN, NM, MM and each of its fields, Snapshot_<grid type, physical
+ * X first>_<Sxy|Vx|Vy|SMM>_<field index on the grid>.txt with the stored
+ * values (no compensation added) times source_scaling, "% 16.15e" per line,
+ * row-major over the grid (data x outer).  Acoustic: P on the MM grid, vx on
+ * NM, vy on MN.  folder must exist. */
+int tw_write_snapshots(tw_solver* s, const char* folder);
 /* Result folder of the reference's device driver
  * (cuda/src/cuda_main/nvcc_device_FDM_2D.cu:755-772):
  * <root>/<elastic|acoustic>/<medium>/device/grid_size_<num>_<den>/Nt_<Nt>/
@@ -364,6 +372,12 @@ int tw_data_misfit(tw_solver* s, const double* data, int32_t nt, double* per_rec
  * and returns the summed durations (ms) and the number of profiled steps. */
 int tw_profile_begin(tw_solver* s, int32_t max_steps);
 int tw_profile_end(tw_solver* s, double* ms_velocity, double* ms_stress, int32_t* nsteps);
+
+/* Device-to-device streaming copy of `bytes` (a multiple of 16, 16-byte
+ * aligned device pointers) on a HIP stream (NULL: the default stream), with
+ * 16-byte non-temporal loads and stores: the attainable copy bandwidth the
+ * benchmark reports beside the solver's.  No counterpart in the reference. */
+int tw_stream_copy(void* dst, const void* src, size_t bytes, void* hip_stream);
 
 /* ---- discrete adjoint (acoustic; FP64 or FP32 storage; no compensation) ----
  *
@@ -395,9 +409,12 @@ int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, 
  * physics, storage, compensation and device; any sizes, materials, sources
  * and boundaries) and advances them together: one launch per half step for
  * the whole batch, the time loop captured in a hipGraph and replayed when the
- * same step range is run again.  Batched steps record no energy and no
- * receiver samples.  The solvers stay usable on their own; calls on them are
- * ordered after the batch's work. */
+ * same step range is run again (a change of source, receivers or material
+ * on any solver rebuilds it).  Batched steps record each solver's energy and
+ * receiver samples as tw_step does (same values; energies summed over the
+ * batch's tile height, so equal to tw_step's to rounding when the heights
+ * differ).  The solvers stay usable on their own; calls on them are ordered
+ * after the batch's work. */
 typedef struct tw_batch tw_batch;
 int tw_batch_create(tw_solver* const* solvers, int32_t n, tw_batch** out);
 int tw_batch_destroy(tw_batch* b);

@@ -169,6 +169,80 @@ def test_batch_graph_rebuilt_after_new_source():
     assert_same(batched, solo)
 
 
+def with_receivers(solvers, seed):
+    for i, s in enumerate(solvers):
+        rng = np.random.default_rng(seed + i)
+        fs, xs, ys = [], [], []
+        for f in s.fields:
+            r, c = s.shape(f)
+            for _ in range(3):
+                fs.append(f)
+                xs.append(int(rng.integers(0, r)))
+                ys.append(int(rng.integers(0, c)))
+        s.set_receivers(fs, xs, ys)
+    return solvers
+
+
+@pytest.mark.parametrize("physics,precision,cpst", [("acoustic", "bf16", "K8"), ("acoustic", "fp64", "0"),
+                                                    ("elastic", "fp32", "6R"), ("elastic", "fp16", "K8S")])
+def test_batch_records_receivers_and_energy(physics, precision, cpst):
+    """Batched steps record every solver's receiver samples and energies as
+    tw_step does: traces bitwise, energies bitwise at the same tile height;
+    graph replay from a restored state records the same again."""
+    print("case batch records", physics, precision, cpst, flush=True)
+    shapes = SHAPES if physics == "acoustic" else [(40, 48, "FFPP"), (33, 70, "PPPP"), (64, 300, "FFFF")]
+    batched, solo = pairs(physics, precision, cpst, shapes)
+    with_receivers(batched, 50)
+    with_receivers(solo, 50)
+    blobs = [s.checkpoint() for s in batched]
+    B = T.Batch(batched)
+    B.step(17)
+    B.step(13)
+    for s in solo:
+        s.step(17)
+        s.step(13)
+    assert_same(batched, solo)
+    for x, y in zip(batched, solo):
+        np.testing.assert_array_equal(x.receivers(), y.receivers())
+        for a, b in zip(x.energy(), y.energy()):
+            np.testing.assert_array_equal(a, b)
+        assert np.abs(x.receivers()).max() > 0 and np.all(np.isfinite(x.energy()[0]))
+    first = [(x.receivers(), x.energy()) for x in batched]
+    for x, blob in zip(batched, blobs):
+        x.restore(blob)
+    B.step(17)  # replays the first graph (same step range, same solvers)
+    B.step(13)
+    for x, (r, e) in zip(batched, first):
+        np.testing.assert_array_equal(x.receivers(), r)
+        for a, b in zip(x.energy(), e):
+            np.testing.assert_array_equal(a, b)
+    B.close()
+
+
+def test_batch_autotune_then_solo_steps_record_energy():
+    """After the autotuner picks a tile height for the batch, each solver's
+    own steps keep their partial-sum layout: a batched range and then solo
+    steps on the same solvers give the energies and traces of solo runs (to
+    rounding for the batched range, whose tile height may differ)."""
+    batched, solo = pairs("acoustic", "fp32", "0")
+    with_receivers(batched, 60)
+    with_receivers(solo, 60)
+    B = T.Batch(batched)
+    B.autotune(4)
+    B.step(10)
+    for s in batched:
+        s.step(6)
+    for s in solo:
+        s.step(16)
+    assert_same(batched, solo)
+    for x, y in zip(batched, solo):
+        np.testing.assert_array_equal(x.receivers(), y.receivers())
+        for a, b in zip(x.energy(), y.energy()):
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=0)
+            np.testing.assert_array_equal(a[11:], b[11:])
+    B.close()
+
+
 def test_batch_autotune_keeps_state():
     batched, solo = pairs("acoustic", "fp32", "0")
     B = T.Batch(batched)

@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 
+import common as C
 import oracle as O
 import twave as T
 import twave.frontend as FE
@@ -113,6 +114,30 @@ def test_run_case_files_match_oracle(tmp_path, with_models):
         e = np.loadtxt(os.path.join(folder, "Energy_src_%d_%s.txt" % (k, tag(sgrid, ix, iy))))
         np.testing.assert_allclose(e, ek + (ep0 + ep1) / 2.0, rtol=1e-12, atol=0)
         assert np.abs(e).max() > 0
+
+
+@pytest.mark.parametrize("physics", ["elastic", "acoustic"])
+def test_write_snapshots(tmp_path, physics):
+    """Snapshot_<type>_<name>_<i>.txt of every field in the reference's grid
+    order and names (nvcc_device_FDM_2D.cu:788-819), values = stored state
+    times source_scaling, "% 16.15e", row-major."""
+    s, o = C.make_pair(physics=physics, mx=24, my=30, bc="FFPP", precision="fp32", cpst="6R", nt=12,
+                       source_scaling=2.0, stencil_form=1 if physics == "acoustic" else 0)
+    C.random_fields(s, o, 3)
+    s.step(12)
+    o.step(12)
+    FE.write_snapshots(s, str(tmp_path))
+    if physics == "elastic":
+        names = {"NN_Sxy_0": T.SXY, "NM_Vx_0": T.VY, "MN_Vy_0": T.VX, "MM_SMM_0": T.SXX, "MM_SMM_1": T.SYY}
+    else:
+        names = {"NM_Vx_0": T.VY, "MN_Vy_0": T.VX, "MM_SMM_0": T.P}
+    files = sorted(p.name for p in tmp_path.iterdir())
+    assert files == sorted("Snapshot_%s.txt" % k for k in names), files
+    for k, f in names.items():
+        got = np.loadtxt(str(tmp_path / ("Snapshot_%s.txt" % k)))
+        want = np.array([float("% 16.15e" % v) for v in (2.0 * o.get_field(f)).ravel()])
+        np.testing.assert_array_equal(got, want, err_msg=k)
+    s.close()
 
 
 def test_write_results_requires_the_shots_receivers(tmp_path):

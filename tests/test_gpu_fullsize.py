@@ -221,3 +221,79 @@ def test_c4_batch_full_sizes():
     B.close()
     for s in batched + solo:
         s.close()
+
+
+def c4_problems(sizes, dx=0.01, vmax=4.5, nt=1600, seed=4, only=None):
+    """Problems of the batched-shots configuration (the kind bench.py --workload c4 runs):
+    n x n cells, smooth random velocity 1.5-4.5 km/s, 10-25 Hz Ricker at a
+    random interior point, characteristic sides, BF16 storage, 16-bit
+    coefficients; returns the solvers (of the indices in `only`, default all:
+    the same problems whatever the selection) and (f0, t0, ix, iy) of each
+    source."""
+    rng = np.random.default_rng(seed)
+    params = []
+    for n in sizes:
+        f0 = float(rng.uniform(10.0, 25.0))
+        ix, iy = (int(v) for v in rng.integers(n // 8, n - n // 8, 2))
+        params.append((f0, ix, iy))
+    out, src = [], []
+    for i, n in enumerate(sizes):
+        if only is not None and i not in only:
+            continue
+        rho, vp, _ = W.smooth_acoustic_model(n, dx, seed=1000 + i)
+        cfg = T.config(physics=T.ACOUSTIC, mx=n, my=n, dx=dx, num_dx=1, den_dx=100,
+                       dt=0.5 * dx / (math.sqrt(2.0) * vmax), bc="CCCC", precision=T.BF16, cpst="0",
+                       coef16=1, nt_max=nt, record_energy=0, stencil_form=1)
+        s = T.Solver(cfg)
+        s.set_model(rho, vp, vp, 1, 100)
+        f0, ix, iy = params[i]
+        s.set_source_ricker(T.GRID_MM, ix, iy, f0, 1.25 / f0)
+        out.append(s)
+        src.append((f0, 1.25 / f0, ix, iy))
+    return out, src
+
+
+def test_c4_batch_as_configured():
+    """The batched-shots configuration as configured: 64 seeded problems with
+    n uniform in [512, 2048], the tile height chosen by the autotuner, the
+    1500-step loop as one graph.  The smallest and the largest problem equal
+    the same problems stepped alone, bit for bit; then a 3-step window of the
+    batch against the CPU oracle started from the smallest problem's state."""
+    print("case C4 as configured", flush=True)
+    rng = np.random.default_rng(4)
+    sizes = [int(v) for v in rng.integers(512, 2049, 64)]
+    K = 1500
+    batched, src = c4_problems(sizes)
+    lo, hi = int(np.argmin(sizes)), int(np.argmax(sizes))
+    solo, _ = c4_problems(sizes, only=(lo, hi))
+    B = T.Batch(batched)
+    rows = B.autotune(20)
+    assert rows in (4, 8, 16, 32, 64)
+    assert all(s.current_step() == 0 for s in batched)
+    B.step(K)
+    for s in solo:
+        s.step(K)
+    for k, s in zip((lo, hi), solo):
+        for f in s.fields:
+            np.testing.assert_array_equal(batched[k].get_field(f), s.get_field(f), err_msg="%d %s" % (sizes[k], f))
+        assert np.any(s.get_field(T.P) != 0)
+    # window: the oracle from the smallest problem's state at step K
+    x = batched[lo]
+    cfg = x.cfg
+    o = O.Oracle(O.make_config(physics=1, mx=cfg.mx, my=cfg.my, dx=cfg.dx, num_dx=cfg.num_dx, den_dx=cfg.den_dx,
+                               dt=cfg.dt, bc="CCCC", precision=O.BF16, cpst="0", coef16=1, nt_max=8,
+                               record_energy=0, stencil_form=1))
+    for k in (T.C_BINV_VX, T.C_BINV_VY, T.C_KAPPA):
+        o.set_coef(k, x.get_coef(k))
+    for f in x.fields:
+        o.set_field(f, x.get_field(f))
+    f0, t0, ix, iy = src[lo]
+    inc = [O.ricker_increment(o.cfg, f0, t0, K + t, 1) for t in range(3)]
+    o.set_source(O.GRID_MM, ix, iy, np.array(inc))
+    B.step(3)
+    o.step(3)
+    for f in x.fields:
+        np.testing.assert_array_equal(x.get_field(f), o.get_field(f), err_msg="window %s" % T.FIELD_NAMES[f])
+    B.close()
+    for s in batched + solo:
+        s.close()

@@ -296,6 +296,7 @@ struct tw_solver {
   std::vector<double> src_inc;
   std::vector<tw::Receiver> rcv_v, rcv_s;
   int nrcv = 0;
+  size_t part_cap = 0;  // doubles in each of plan.part_v / part_s
   int it = 0;
   // bumped by every change that a batch's captured tables depend on (source,
   // receivers, coefficients): a batch rebuilds its graph when it differs
@@ -672,6 +673,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     return bail(e);
   p.nparts = p.fused ? tw::fused_parts(p) : tw::half_parts(p);
   const size_t npart_max = (size_t)std::max(p.nparts, tw::energy_parts(p));
+  s->part_cap = npart_max;
   if ((e = dalloc(s, (void**)&p.part_v, npart_max * 8)) || (e = dalloc(s, (void**)&p.part_s, npart_max * 8)))
     return bail(e);
   // device copies of the static kernel arguments: in place, and (fused step)
@@ -1308,6 +1310,14 @@ int tw_check_finite(tw_solver* s) {
   return TW_OK;
 }
 
+int tw_stream_copy(void* dst, const void* src, size_t bytes, void* hip_stream) {
+  if (!dst || !src || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+    return fail(nullptr, TW_EINVAL, "stream copy: 16-byte aligned pointers and a multiple of 16 bytes");
+  const hipError_t e = tw::launch_stream_copy(dst, src, bytes, (hipStream_t)hip_stream);
+  if (e != hipSuccess) return fail(nullptr, TW_EDEVICE, std::string("stream copy: ") + hipGetErrorString(e));
+  return TW_OK;
+}
+
 int tw_check_finite_noalloc(tw_solver* s) {
   if (!s) return fail(nullptr, TW_EINVAL, "null handle");
   (void)hipSetDevice(s->cfg.device);
@@ -1607,6 +1617,7 @@ struct tw_batch {
   int g_nsteps = -1, g_rows = -1;
   std::vector<int> g_it0;
   std::vector<uint64_t> g_gen;  // each solver's generation when the graph was built
+  std::vector<char> g_ep;       // each solver's ep_valid before the graph was built
   void* items = nullptr;
   size_t items_cap = 0;
   // per step and half: item offsets and launch sizes
@@ -1649,24 +1660,103 @@ tw::Dyn batch_dyn(tw_solver* s, int it, int half) {
 }
 
 // item tables of nsteps steps from each solver's current step, in device
-// memory, and the graph that launches them
+// memory, and the graph that launches them.  Energies and receiver samples are
+// recorded as tw_step records them: each launch's first block of a problem
+// reduces that problem's energy partials of the previous launch and samples
+// the receivers of the fields it finished; the stress energy of the start
+// state (when not known) and the last step's pending work are launches of
+// their own per problem, inside the graph.
 int batch_build(tw_batch* b, int nsteps) {
   const int n = (int)b->sv.size();
   std::vector<tw::BatchCore> core;
   std::vector<tw::BatchBand> band;
   b->halves.clear();
+  // the batch's tile height for the geometry; each solver's own is restored
+  // below, so its single-problem steps keep their partial-sum layout
+  std::vector<int> txc0(n), nparts(n), has_core(n);
+  for (int k = 0; k < n; k++) {
+    tw_solver* s = b->sv[k];
+    txc0[k] = s->plan.txc;
+    s->plan.txc = b->rows;
+    nparts[k] = tw::half_parts(s->plan);
+    tw::BatchCore c{};
+    tw::BatchBand r{};
+    int nc, nb;
+    tw::batch_geometry(s->plan, &c, &nc, &r, &nb);
+    has_core[k] = nc > 0;
+  }
+  auto restore_txc = [&]() {
+    for (int k = 0; k < n; k++) b->sv[k]->plan.txc = txc0[k];
+  };
+  for (int k = 0; k < n; k++) {
+    tw_solver* s = b->sv[k];
+    const tw_config* c = &s->cfg;
+    const bool records = (c->record_energy || s->nrcv > 0) && s->it < c->nt_max;
+    if (records && !has_core[k]) {
+      restore_txc();
+      return bfail(b, TW_EINVAL, "batch: a problem without core rows cannot record energy or receivers");
+    }
+    // energy partials of the batch's launch geometry
+    const size_t need = (size_t)std::max(nparts[k], tw::energy_parts(s->plan));
+    if (c->record_energy && need > s->part_cap) {
+      dfree(s, (void**)&s->plan.part_v);
+      dfree(s, (void**)&s->plan.part_s);
+      int e;
+      if ((e = dalloc(s, (void**)&s->plan.part_v, need * 8)) || (e = dalloc(s, (void**)&s->plan.part_s, need * 8))) {
+        restore_txc();
+        return bfail(b, e, "batch: energy partial buffers: " + s->err);
+      }
+      s->part_cap = need;
+    }
+  }
+  std::vector<tw::Pending> carry(n);
+  std::vector<char> have(n, 0), ep0(n);
+  for (int k = 0; k < n; k++) ep0[k] = b->sv[k]->ep_valid;
   for (int t = 0; t < nsteps; t++)
     for (int half = 0; half < 2; half++) {
       tw_batch::Half H{core.size(), band.size(), 0, 0, 0, 0};
       for (int k = 0; k < n; k++) {
         tw_solver* s = b->sv[k];
-        s->plan.txc = b->rows;
+        const tw_config* cf = &s->cfg;
+        const tw::Plan& pl = s->plan;
+        const int it = s->it + t;
+        const bool rec = cf->record_energy && it < cf->nt_max;
         tw::BatchCore c{};
         tw::BatchBand r{};
         int nc, nb;
-        tw::batch_geometry(s->plan, &c, &nc, &r, &nb);
-        c.sp = r.sp = s->plan.dstat;
-        c.d = r.d = batch_dyn(s, s->it + t, half);
+        tw::batch_geometry(pl, &c, &nc, &r, &nb);
+        c.sp = r.sp = pl.dstat;
+        tw::Dyn d = batch_dyn(s, it, half);
+        d.energy = rec;
+        if (half == 0) {
+          if (t == 0 && rec && !s->ep_valid) {  // stress energy of the start state, launched first
+            carry[k] = tw::Pending();
+            carry[k].part = pl.part_s;
+            carry[k].nparts = tw::energy_parts(pl);
+            carry[k].out0 = pl.ep0;
+            carry[k].it0 = it;
+            carry[k].scale = cf->energy_form == 1 ? 1.0 : 0.5;
+            have[k] = 1;
+          }
+          if (have[k]) d.pend = carry[k];
+        } else {
+          if (rec) {
+            d.pend.part = pl.part_v;
+            d.pend.nparts = nparts[k];
+            d.pend.out0 = pl.ek;
+            d.pend.it0 = it;
+            d.pend.scale = cf->energy_form == 1 ? 1.0 : 0.5;
+          }
+          if (it < cf->nt_max && !s->rcv_v.empty()) {
+            d.pend.rcv = pl.rcv_v;
+            d.pend.nrcv = (int32_t)s->rcv_v.size();
+            d.pend.rcv_it = it;
+          }
+          carry[k] = stress_pending(s, it, rec);
+          carry[k].nparts = rec ? nparts[k] : 0;
+          have[k] = 1;
+        }
+        c.d = r.d = d;
         if (nc > 0) {
           c.b0 = H.core_blocks;
           H.core_blocks += nc;
@@ -1676,6 +1766,8 @@ int batch_build(tw_batch* b, int nsteps) {
         if (nb > 0) {
           r.b0 = H.band_blocks;
           H.band_blocks += nb;
+          // the problem's core blocks do its pending work; band tiles never do
+          r.d.pend = tw::Pending();
           band.push_back(r);
           H.nband++;
         }
@@ -1686,40 +1778,58 @@ int batch_build(tw_batch* b, int nsteps) {
   if (b->items_cap < cb + bb) {
     if (b->items) (void)hipFree(b->items);
     b->items = nullptr;
-    BHIP(b, hipMalloc(&b->items, cb + bb));
+    if (hipMalloc(&b->items, cb + bb) != hipSuccess) {
+      restore_txc();
+      return bfail(b, TW_ENOMEM, "batch: item tables");
+    }
     b->items_cap = cb + bb;
   }
-  BHIP(b, hipMemcpyAsync(b->items, core.data(), cb, hipMemcpyHostToDevice, b->stream));
-  BHIP(b, hipMemcpyAsync((char*)b->items + cb, band.data(), bb, hipMemcpyHostToDevice, b->stream));
-  BHIP(b, hipStreamSynchronize(b->stream));
+  hipError_t he = hipMemcpyAsync(b->items, core.data(), cb, hipMemcpyHostToDevice, b->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync((char*)b->items + cb, band.data(), bb, hipMemcpyHostToDevice, b->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(b->stream);
+  if (he != hipSuccess) {
+    restore_txc();
+    return bfail(b, TW_EDEVICE, std::string("batch tables: ") + hipGetErrorString(he));
+  }
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   b->exec = nullptr;
   const tw::BatchCore* dc = (const tw::BatchCore*)b->items;
   const tw::BatchBand* db = (const tw::BatchBand*)((char*)b->items + cb);
   const Plan& p0 = b->sv[0]->plan;
   hipGraph_t g = nullptr;
-  BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < (int)b->halves.size(); i++) {
-    const tw_batch::Half& H = b->halves[i];
-    hipError_t e = tw::launch_batch_half(p0, i % 2, dc + H.core_off, H.ncore, H.core_blocks, db + H.band_off, H.nband,
-                                         H.band_blocks, b->stream);
-    if (e != hipSuccess) {
-      (void)hipStreamEndCapture(b->stream, &g);
-      if (g) (void)hipGraphDestroy(g);
-      return bfail(b, TW_EDEVICE, std::string("batched launch: ") + hipGetErrorString(e));
-    }
+  he = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+  for (int k = 0; k < n && he == hipSuccess; k++) {
+    tw_solver* s = b->sv[k];
+    if (s->cfg.record_energy && s->it < s->cfg.nt_max && !s->ep_valid) he = tw::launch_stress_energy(s->plan, b->stream);
   }
-  BHIP(b, hipStreamEndCapture(b->stream, &g));
-  hipError_t e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) return bfail(b, TW_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  for (int i = 0; i < (int)b->halves.size() && he == hipSuccess; i++) {
+    const tw_batch::Half& H = b->halves[i];
+    he = tw::launch_batch_half(p0, i % 2, dc + H.core_off, H.ncore, H.core_blocks, db + H.band_off, H.nband,
+                               H.band_blocks, b->stream);
+  }
+  for (int k = 0; k < n && he == hipSuccess; k++)
+    if (have[k]) he = tw::launch_finalize(b->sv[k]->plan, carry[k], b->stream);
+  if (he != hipSuccess) {
+    (void)hipStreamEndCapture(b->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    restore_txc();
+    return bfail(b, TW_EDEVICE, std::string("batched launch: ") + hipGetErrorString(he));
+  }
+  he = hipStreamEndCapture(b->stream, &g);
+  if (he == hipSuccess) he = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  restore_txc();
+  if (he != hipSuccess) return bfail(b, TW_EDEVICE, std::string("batch graph: ") + hipGetErrorString(he));
   b->g_nsteps = nsteps;
   b->g_rows = b->rows;
   b->g_it0.clear();
   b->g_gen.clear();
-  for (tw_solver* s : b->sv) {
-    b->g_it0.push_back(s->it);
-    b->g_gen.push_back(s->gen);
+  b->g_ep.clear();
+  for (int k = 0; k < n; k++) {
+    b->g_it0.push_back(b->sv[k]->it);
+    b->g_gen.push_back(b->sv[k]->gen);
+    b->g_ep.push_back(ep0[k]);
   }
   return TW_OK;
 }
@@ -1727,7 +1837,8 @@ int batch_build(tw_batch* b, int nsteps) {
 bool batch_cached(const tw_batch* b, int nsteps) {
   if (!b->exec || b->g_nsteps != nsteps || b->g_rows != b->rows) return false;
   for (size_t k = 0; k < b->sv.size(); k++)
-    if (b->g_it0[k] != b->sv[k]->it || b->g_gen[k] != b->sv[k]->gen) return false;
+    if (b->g_it0[k] != b->sv[k]->it || b->g_gen[k] != b->sv[k]->gen || b->g_ep[k] != (char)b->sv[k]->ep_valid)
+      return false;
   return true;
 }
 
@@ -1744,8 +1855,10 @@ int batch_launch(tw_batch* b, int nsteps, hipEvent_t t0, hipEvent_t t1) {
   BHIP(b, hipEventRecord(b->done, b->stream));
   for (tw_solver* s : b->sv) {
     BHIP(b, hipStreamWaitEvent(s->stream, b->done, 0));
+    // the graph carried the stress energy of each recorded step into E_p0 of
+    // the next, as tw_step does
+    if (s->cfg.record_energy && s->it < s->cfg.nt_max) s->ep_valid = true;
     s->it += nsteps;
-    s->ep_valid = false;
   }
   return TW_OK;
 }
@@ -1890,7 +2003,6 @@ int tw_batch_autotune(tw_batch* b, int32_t trial_steps, int32_t* best_rows) {
   (void)hipEventDestroy(t1);
   if (e) return e;
   b->rows = best_r;
-  for (tw_solver* s : b->sv) s->plan.txc = best_r;
   if (best_rows) *best_rows = best_r;
   return TW_OK;
 }

@@ -232,6 +232,7 @@ hipError_t launch_uhat(const Plan& p, hipStream_t s);
 hipError_t launch_stress_energy(const Plan& p, hipStream_t s);
 hipError_t launch_finalize(const Plan& p, const Pending& pend, hipStream_t s);
 hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s);
+hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // allocation-free finiteness check (tw_noalloc.hip): device address of the
 // per-device scratch array (resolved once per device, the device current),
 // and the check with its flag there, copied to *host_flag on stream s

@@ -456,6 +456,35 @@ int tw_write_results(tw_solver* s, const char* folder, const tw_srcrcv_entry* en
   return TW_OK;
 }
 
+int tw_write_snapshots(tw_solver* s, const char* folder) {
+  if (!s || !folder) return err(TW_EINVAL, "null argument");
+  tw_config cfg;
+  int e = tw_get_config(s, &cfg);
+  if (e) return e;
+  // grids in the reference's order NN, MN, NM, MM; names "Sxy", "Vx", "Vy",
+  // "SMM" (data axes, grid.hpp:155-159); the file name gives the type with
+  // physical X first; elastic fields per grid in soln order, acoustic as the
+  // single field of its grid
+  static const char* kName[4] = {"Sxy", "Vx", "Vy", "SMM"};
+  static const char kType[4][2] = {{'N', 'N'}, {'M', 'N'}, {'N', 'M'}, {'M', 'M'}};
+  for (int g = 0; g < 4; g++) {
+    const std::vector<int32_t> fs = grid_fields(cfg.physics, g);
+    for (size_t i = 0; i < fs.size(); i++) {
+      int32_t rows = 0, cols = 0;
+      if ((e = tw_field_shape(&cfg, fs[i], &rows, &cols))) return e;
+      std::vector<double> v((size_t)rows * cols);
+      if ((e = tw_get_field(s, fs[i], v.data()))) return e;
+      const std::string path = std::string(folder) + "/Snapshot_" + kType[g][1] + kType[g][0] + "_" + kName[g] + "_" +
+                               std::to_string(i) + ".txt";
+      FILE* f = fopen(path.c_str(), "w");
+      if (!f) return err(TW_EINVAL, "cannot write " + path);
+      for (size_t k = 0; k < v.size(); k++) fprintf(f, "% 16.15e\n", cfg.source_scaling * v[k]);
+      fclose(f);
+    }
+  }
+  return TW_OK;
+}
+
 int tw_result_folder(const tw_input* in, const tw_config* cfg, const char* root, char* out, int32_t cap) {
   if (!in || !cfg || !root || !out || cap <= 0) return err(TW_EINVAL, "bad argument");
   static const char* prec[4] = {"fp64", "fp32", "bf16", "fp16"};

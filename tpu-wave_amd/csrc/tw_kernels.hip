@@ -2549,6 +2549,26 @@ hipError_t TW_PUB(launch_uhat)(const Plan& p, hipStream_t s) { return misc(p, 1,
 hipError_t TW_PUB(launch_finalize)(const Plan& p, const Pending& pend, hipStream_t s) { return misc(p, 2, &pend, s); }
 
 #if TW_PREC == 0
+// Streaming copy (16-byte non-temporal loads and stores, grid-stride): the
+// attainable read + write bandwidth of a kernel that moves every byte once,
+// which the benchmark reports beside the solver's.
+__global__ void __launch_bounds__(256) k_copy16(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    typedef __attribute__((address_space(1))) const u32x4* Q;
+    typedef __attribute__((address_space(1))) u32x4* W;
+    const u32x4 v = __builtin_nontemporal_load((Q)(src + i));
+    __builtin_nontemporal_store(v, (W)(dst + i));
+  }
+}
+
+hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  const size_t want = (n + 255) / 256;
+  const unsigned blocks = (unsigned)(want < 16384 ? (want > 0 ? want : 1) : 16384);
+  hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, (const u32x4*)src, (u32x4*)dst, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_check_finite(const Plan& p, int32_t* flag, hipStream_t s) {
   dim3 b(64, 4);
   for (int f = 0; f < NF; f++) {

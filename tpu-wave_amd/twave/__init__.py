@@ -137,6 +137,7 @@ _SIGS = [
     ("tw_set_stream", ctypes.c_int, [_vp, _vp]),
     ("tw_check_finite", ctypes.c_int, [_vp]),
     ("tw_check_finite_noalloc", ctypes.c_int, [_vp]),
+    ("tw_stream_copy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     ("tw_get_config", ctypes.c_int, [_vp, _cp]),
     ("tw_input_default", None, [ctypes.POINTER(Input)]),
     ("tw_read_input_file", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(Input)]),
@@ -146,6 +147,7 @@ _SIGS = [
     ("tw_read_model_file", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, _dp]),
     ("tw_setup_shot", ctypes.c_int, [_vp, ctypes.POINTER(Input), ctypes.POINTER(SrcRcvEntry), ctypes.c_int32,
                                      ctypes.c_int32]),
+    ("tw_write_snapshots", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("tw_write_results", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(SrcRcvEntry), ctypes.c_int32,
                                         ctypes.c_int32]),
     ("tw_result_folder", ctypes.c_int, [ctypes.POINTER(Input), _cp, ctypes.c_char_p, ctypes.c_char_p,

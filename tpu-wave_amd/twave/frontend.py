@@ -75,6 +75,11 @@ def write_results(solver, folder, entries, n, src_index):
     solver._chk(lib().tw_write_results(solver.h, os.fsencode(folder), entries, n, src_index))
 
 
+def write_snapshots(solver, folder):
+    """Final-state Snapshot_*.txt files of every field (tw_write_snapshots)."""
+    solver._chk(lib().tw_write_snapshots(solver.h, os.fsencode(folder)))
+
+
 def run_case(input_file, srcrcv_file, result_root, options=None, precision="fp32", cpst="0", model_dir=None,
              reference_boundaries=True, sources=None):
     """Run every source of a SrcRcv list as the reference's host driver does

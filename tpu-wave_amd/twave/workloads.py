@@ -156,6 +156,32 @@ class AdjointShots:
         tr = s.receivers()[:, -self.nt:]
         return tr, s.adjoint_run(2.0 * (tr - residual_of))
 
+    def misfit_batched(self, coefs):
+        """J for the coefficients with all shots advanced together as one
+        batch (one solver per shot, one launch per half step for all of
+        them, the step loop in a hipGraph); the same traces as misfit()."""
+        from . import Batch, Solver
+        if getattr(self, "_shot_solvers", None) is None:
+            self._shot_solvers = []
+            for i in range(len(self.shots)):
+                s = Solver(self.s.cfg)
+                f, x, y = self.s.get_receivers()
+                s.set_receivers(f, x, y)
+                s.set_source_ricker(GRID_MM, 4, self.shots[i], self.f0, self.t0)
+                self._shot_solvers.append(s)
+            self._shot_state0 = [s.checkpoint() for s in self._shot_solvers]
+            self._batch = Batch(self._shot_solvers)
+        for s, blob in zip(self._shot_solvers, self._shot_state0):
+            s.restore(blob)
+            for k, v in zip(self.coefs, coefs):
+                s.set_coef(k, v)
+        self._batch.step(self.nt)
+        total = 0.0
+        for s, d in zip(self._shot_solvers, self.data):
+            tr = s.receivers()[:, -self.nt:]
+            total += float(np.sum((tr - d) ** 2))
+        return total
+
     def misfit(self, coefs, grad=False):
         """J and (with grad) its gradient, summed over the shots."""
         total, g = 0.0, [np.zeros_like(m) for m in self.m0]
@@ -168,6 +194,11 @@ class AdjointShots:
         return total, g
 
     def close(self):
+        if getattr(self, "_batch", None) is not None:
+            self._batch.close()
+            for s in self._shot_solvers:
+                s.close()
+            self._batch, self._shot_solvers = None, None
         self.s.close()
 
 
