@@ -47,13 +47,14 @@ def comp_bytes(s, cpst):
 
 def coef_bytes(precision, coef16):
     """Bytes per stored material coefficient: FP64 with FP64 storage, else
-    FP32, the 16-bit type (coef16 1) or an 8-bit code (coef16 2; the per-row
-    offset and step add 8 bytes per row, under 0.01 B per cell)."""
+    FP32, the 16-bit type (coef16 1), an 8-bit code (coef16 2) or a 16-bit
+    code (coef16 3; the per-row offset and step of the codes add 8 bytes per
+    row, under 0.01 B per cell)."""
     if precision == 0:
         return 8
     if precision == 1:
         return 4
-    return {0: 4, 1: 2, 2: 1}[coef16]
+    return {0: 4, 1: 2, 2: 1, 3: 2}[coef16]
 
 
 def algorithmic_bytes(precision, cpst, coef16, physics=0):
@@ -107,23 +108,33 @@ def measured_traffic(key, kernel):
     return None
 
 
-def copy_calibration(torch, nbytes, reps):
+def copy_calibration(torch, nbytes, reps, kernel="torch"):
     """Plain device copy of nbytes, reps times: the known-size kernel that
     scales the FETCH_SIZE / WRITE_SIZE counters, and the attainable
-    read+write bandwidth of a streaming copy on this device."""
+    read+write bandwidth of a streaming copy on this device.  kernel "torch":
+    torch's copy; "tw": the library's streaming copy with non-temporal loads
+    and stores (the load kind the solver's once-read rows use)."""
     a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
     b = torch.empty_like(a)
-    b.copy_(a)
+    st = torch.cuda.current_stream()
+    if kernel == "tw":
+        import twave as T
+        L = T.lib()
+        run = lambda: L.tw_stream_copy(b.data_ptr(), a.data_ptr(), nbytes, st.cuda_stream)
+    else:
+        run = lambda: b.copy_(a)
+    run()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        b.copy_(a)
+        run()
     e1.record()
     torch.cuda.synchronize()
     t = e0.elapsed_time(e1) / 1e3 / reps
-    print(json.dumps({"copy_bytes": nbytes, "reps": reps + 1, "copy_read_plus_write_GBps": 2 * nbytes / t / 1e9}))
+    print(json.dumps({"copy_kernel": kernel, "copy_bytes": nbytes, "reps": reps + 1,
+                      "copy_read_plus_write_GBps": 2 * nbytes / t / 1e9}))
 
 
 def stream_copy_bandwidth(torch, T, nbytes=1 << 30, reps=20):
@@ -381,18 +392,20 @@ def main():
                     "config), K = 16-bit residual, 0 = none")
     ap.add_argument("--coef16", type=int, default=2,
                     help="material coefficient storage: 0 FP32, 1 the 16-bit type, 2 one byte per value with a "
-                    "per-row affine map (default)")
+                    "per-row affine map (default), 3 two bytes per value with a per-row affine map")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32, FP64 and variant runs")
     ap.add_argument("--accuracy-steps", type=int, default=2000,
                     help="step count at which every run is compared with FP64 (untimed beyond the timed range; "
                     "the configuration's 2000 steps by default)")
-    ap.add_argument("--variants", default="bf16:K8:1,bf16:0:2,fp16:K8:2",
+    ap.add_argument("--variants", default="bf16:K8:1,bf16:0:2,fp16:K8:2,bf16:K8S:3,fp16:K8S:3",
                     help="comma-separated storage:cpst[:coef16] triples (coef16 default 1) timed and checked "
                     "against FP64 beside the main run, on the same material")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-launch events in the timed region (roofline from the step time)")
     ap.add_argument("--cpu-size", type=int, default=0, help="grid of the CPU sample (default: --size)")
+    ap.add_argument("--copy-kernel", default="torch", choices=["torch", "tw"],
+                    help="copy of --copy-calib: torch's, or the library's non-temporal streaming copy")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     ap.add_argument("--workload", default="elastic", choices=["elastic", "c4"],
@@ -408,7 +421,7 @@ def main():
         sys.exit(2)
     import torch
     if args.copy_calib:
-        copy_calibration(torch, args.copy_calib, 5)
+        copy_calibration(torch, args.copy_calib, 5, args.copy_kernel)
         return
     if args.workload == "c4":
         c4_batch(args)

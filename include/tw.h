@@ -96,7 +96,9 @@ typedef struct tw_config {
                              0: FP32; 1: the 16-bit storage type; 2: one byte
                              per value, each row mapped affinely onto codes
                              0..255 (FP32 offset = row minimum, step = row
-                             range / 255, value = offset + code * step in FP32).
+                             range / 255, value = offset + code * step in FP32);
+                             3: the same with two-byte codes 0..65535 (step =
+                             row range / 65535).
                              tw_get_coef returns the values the kernels use.   */
   int32_t cpst;           /* tw_cpst                                                 */
   int32_t stencil_form;   /* 0: stencil*(dt/dx) (cuda/src/cpp/grid.hpp:281-282);

@@ -52,14 +52,16 @@ def test_config_default_and_check():
         T.check_config(T.config(precision="bf16", cpst=cp))
     with pytest.raises(T.TwError):
         T.check_config(T.config(precision="bf16", cpst="6R"))
-    # coefficient storage: 8-bit codes only with 16-bit state, nothing beyond 2
+    # coefficient storage: 8- and 16-bit codes only with 16-bit state, nothing beyond 3
     for prec in ("bf16", "fp16"):
-        T.check_config(T.config(precision=prec, cpst="K8", coef16=2))
+        for codes in (2, 3):
+            T.check_config(T.config(precision=prec, cpst="K8", coef16=codes))
     for prec in ("fp32", "fp64"):
-        with pytest.raises(T.TwError):
-            T.check_config(T.config(precision=prec, coef16=2))
+        for codes in (2, 3):
+            with pytest.raises(T.TwError):
+                T.check_config(T.config(precision=prec, coef16=codes))
     with pytest.raises(T.TwError):
-        T.check_config(T.config(precision="bf16", coef16=3))
+        T.check_config(T.config(precision="bf16", coef16=4))
     with pytest.raises(T.TwError):
         T.check_config(T.config(energy_form=1))    # acoustic periodic only
     with pytest.raises(T.TwError) as e:

@@ -88,7 +88,7 @@ def oracle_twin(physics, precision, cpst, mx, my, bc, seed, nt=200, coef16=0):
     coefficients the solver decoded (8-bit codes: FP32 coefficients set to the
     decoded values), the same seeded fields and source."""
     kw = C.cfg_kwargs(physics=physics, mx=mx, my=my, bc=bc, precision=precision, cpst=cpst, nt=nt,
-                      stencil_form=1 if physics == "acoustic" else 0, coef16=0 if coef16 == 2 else coef16)
+                      stencil_form=1 if physics == "acoustic" else 0, coef16=0 if coef16 >= 2 else coef16)
     o = O.Oracle(O.make_config(**kw))
     s = problem(physics, precision, cpst, mx, my, bc, seed, nt=nt, coef16=coef16)
     ks = [T.C_BINV_VX, T.C_BINV_VY, T.C_KAPPA] if physics == "acoustic" else \
@@ -105,7 +105,8 @@ def oracle_twin(physics, precision, cpst, mx, my, bc, seed, nt=200, coef16=0):
 
 @pytest.mark.parametrize("physics,precision,cpst,coef16", [("acoustic", "bf16", "K8", 1), ("acoustic", "bf16", "0", 2),
                                                            ("elastic", "bf16", "K8", 2), ("acoustic", "fp32", "6R", 0),
-                                                           ("elastic", "fp16", "K8S", 2), ("acoustic", "bf16", "K8S", 1)])
+                                                           ("elastic", "fp16", "K8S", 2), ("acoustic", "bf16", "K8S", 1),
+                                                           ("elastic", "bf16", "K8S", 3)])
 def test_batch_equals_oracle(physics, precision, cpst, coef16):
     """Batched launches against the CPU oracle directly (not only against the
     same problems stepped alone): every field and companion bitwise after a

@@ -129,25 +129,27 @@ def test_k8s_residual_fed(physics, bc, precision, coef16, mx, my):
              stencil_form=1 if physics == "acoustic" else 0)
 
 
+@pytest.mark.parametrize("codes", [2, 3])
 @pytest.mark.parametrize("physics,bc,precision,cpst,mx,my", [
     ("elastic", "FCCC", "bf16", "K8", 40, 48), ("elastic", "FFPP", "fp16", "K8", 40, 48),
     ("elastic", "FCCC", "bf16", "K8S", 40, 48), ("acoustic", "CCCC", "fp16", "K8S", 40, 48),
     ("elastic", "PPPP", "bf16", "0", 40, 48), ("elastic", "FCFC", "bf16", "K8", 21, 531),
     ("acoustic", "CCCC", "bf16", "0", 40, 48), ("acoustic", "FCPP", "fp16", "K8", 40, 48),
     ("acoustic", "PPCC", "bf16", "K", 17, 517)])
-def test_coef8(physics, bc, precision, cpst, mx, my):
-    """8-bit material coefficients (coef16 = 2): the solver quantizes each row
-    onto 256 codes of an affine map; the oracle runs FP32 coefficients set to
-    the values the solver reports (tw_get_coef), which are what its kernels
-    decode, so fields and traces agree bit for bit.  The quantization itself:
-    every value within half a step (row range / 255) of what was set, at most
-    256 distinct values per row."""
-    print("case coef8", physics, bc, precision, cpst, mx, my, flush=True)
+def test_coef8(physics, bc, precision, cpst, mx, my, codes):
+    """Material coefficients as codes of a per-row affine map, 8-bit (coef16 =
+    2, 256 codes) or 16-bit (coef16 = 3, 65536 codes): the oracle runs FP32
+    coefficients set to the values the solver reports (tw_get_coef), which are
+    what its kernels decode, so fields and traces agree bit for bit.  The
+    quantization itself: every value within half a step (row range / levels)
+    of what was set, at most levels + 1 distinct values per row."""
+    print("case coef codes", codes, physics, bc, precision, cpst, mx, my, flush=True)
+    levels = 255 if codes == 2 else 65535
     import oracle as O
     ac = physics == "acoustic"
     kw = C.cfg_kwargs(physics=physics, mx=mx, my=my, bc=bc, precision=precision, cpst=cpst, nt=30,
                       stencil_form=1 if ac else 0)
-    s = T.Solver(T.config(**dict(kw, coef16=2)))
+    s = T.Solver(T.config(**dict(kw, coef16=codes)))
     o = O.Oracle(O.make_config(**dict(kw, coef16=0)))
     rng = np.random.default_rng(7)
     ks = [T.C_BINV_VX, T.C_BINV_VY, T.C_KAPPA] if ac else [T.C_BINV_VX, T.C_BINV_VY, T.C_MU, T.C_LAMBDA, T.C_TWOMU]
@@ -157,9 +159,9 @@ def test_coef8(physics, bc, precision, cpst, mx, my):
         v = rng.uniform(lo, hi, shape)
         s.set_coef(k, v)
         d = s.get_coef(k)
-        step = (v.max(axis=1) - v.min(axis=1)) / 255.0
+        step = (v.max(axis=1) - v.min(axis=1)) / levels
         assert np.all(np.abs(d - v) <= 0.5 * step[:, None] * (1 + 1e-5) + 1e-6 * np.abs(v))
-        assert all(len(np.unique(row)) <= 256 for row in d)
+        assert all(len(np.unique(row)) <= levels + 1 for row in d)
         o.set_coef(k, d)
     C.random_fields(s, o, 8)
     C.receivers_all(s, o, 9)

@@ -115,7 +115,7 @@ size_t array_size(const tw_config* c, int which) {
 size_t coef_size(int precision, int coef16) {
   if (precision == TW_FP64) return 8;
   if (precision == TW_FP32) return 4;
-  return coef16 == 2 ? 1 : (coef16 ? 2 : 4);
+  return coef16 == 2 ? 1 : (coef16 ? 2 : 4);  // 3: 16-bit codes
 }
 size_t arith_size(int precision) { return precision == TW_FP64 ? 8 : 4; }
 
@@ -150,7 +150,7 @@ void from_storage(int precision, const void* in, double* out, size_t n) {
 }
 // coefficient rounding: its storage type (the reference casts to the solver precision)
 double round_coef(const tw_config* c, double v) {
-  if (c->coef16 == 2 && !ref_precision(c->precision)) return v;  // quantized per row at upload
+  if (c->coef16 >= 2 && !ref_precision(c->precision)) return v;  // quantized per row at upload
   switch (c->precision) {
     case TW_FP64: return v;
     case TW_FP32: return (double)(float)v;
@@ -185,9 +185,9 @@ int check_config(const tw_config* c, std::string* err) {
   if (c->energy_form == 1 && (c->physics != TW_ACOUSTIC || c->bc[0] != 'P' || c->bc[2] != 'P'))
     return fail(err, TW_EINVAL, "energy_form 1 is the periodic acoustic reference form");
   if (c->energy_form != 0 && c->energy_form != 1) return fail(err, TW_EINVAL, "energy_form");
-  if (c->coef16 < 0 || c->coef16 > 2) return fail(err, TW_EINVAL, "coef16 must be 0, 1 or 2");
-  if (c->coef16 == 2 && ref_precision(c->precision))
-    return fail(err, TW_EINVAL, "8-bit coefficients (coef16 2) need 16-bit storage");
+  if (c->coef16 < 0 || c->coef16 > 3) return fail(err, TW_EINVAL, "coef16 must be 0, 1, 2 or 3");
+  if (c->coef16 >= 2 && ref_precision(c->precision))
+    return fail(err, TW_EINVAL, "coefficient codes (coef16 2, 3) need 16-bit storage");
   if (c->nt_max < 0) return fail(err, TW_EINVAL, "nt_max must be >= 0");
   if (!(c->source_scaling > 0)) return fail(err, TW_EINVAL, "source_scaling must be > 0");
   return TW_OK;
@@ -350,31 +350,33 @@ void dfree(tw_solver* s, void** p) {
   *p = nullptr;
 }
 
-// 8-bit coefficients: each row is mapped affinely onto [0, 255] (offset =
-// the row minimum, step = range / 255, both FP32), every value takes the code
-// whose FP32 decode off + q * step is nearest, and the host copy is replaced
-// by the decoded values, which is what the kernels compute with.
-void quantize_row8(double* v, int n, uint8_t* q, float* off, float* step) {
+// Coefficient codes: each row is mapped affinely onto [0, levels] (levels
+// 255 for 8-bit, 65535 for 16-bit codes; offset = the row minimum, step =
+// range / levels, both FP32), every value takes the code whose FP32 decode
+// off + q * step is nearest, and the host copy is replaced by the decoded
+// values, which is what the kernels compute with.
+template <class Q>
+void quantize_row(double* v, int n, Q* q, float* off, float* step, int levels) {
   double lo = v[0], hi = v[0];
   for (int i = 1; i < n; i++) {
     lo = std::min(lo, v[i]);
     hi = std::max(hi, v[i]);
   }
   const float o = (float)lo;
-  const float st = (float)((hi - lo) / 255.0);
+  const float st = (float)((hi - lo) / (double)levels);
   for (int i = 0; i < n; i++) {
     int c0 = st > 0 ? (int)std::lround((v[i] - (double)o) / (double)st) : 0;
     int best = 0;
     double err = 0.0;
     for (int c = c0 - 1; c <= c0 + 1; c++) {
-      const int cc = std::min(255, std::max(0, c));
+      const int cc = std::min(levels, std::max(0, c));
       const double e = std::fabs((double)tw::c8_dec((unsigned)cc, o, st) - v[i]);
       if (c == c0 - 1 || e < err) {
         best = cc;
         err = e;
       }
     }
-    q[i] = (uint8_t)best;
+    q[i] = (Q)best;
     v[i] = (double)tw::c8_dec((unsigned)best, o, st);
   }
   *off = o;
@@ -388,9 +390,19 @@ int upload_coef(tw_solver* s, int k) {
     std::vector<uint8_t> codes((size_t)d.rows * d.pitch, 0);
     std::vector<float> rp((size_t)d.rows * 2);
     for (int r = 0; r < d.rows; r++)
-      quantize_row8(s->hcoef[k].data() + (size_t)r * d.cols, d.cols, codes.data() + (size_t)r * d.pitch, &rp[2 * r],
-                    &rp[2 * r + 1]);
+      quantize_row(s->hcoef[k].data() + (size_t)r * d.cols, d.cols, codes.data() + (size_t)r * d.pitch, &rp[2 * r],
+                   &rp[2 * r + 1], 255);
     HIPCHK(s, hipMemcpy(d.p, codes.data(), codes.size(), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.rp, rp.data(), rp.size() * sizeof(float), hipMemcpyHostToDevice));
+    return TW_OK;
+  }
+  if (c->coef16 == 3 && !ref_precision(c->precision)) {
+    std::vector<uint16_t> codes((size_t)d.rows * d.pitch, 0);
+    std::vector<float> rp((size_t)d.rows * 2);
+    for (int r = 0; r < d.rows; r++)
+      quantize_row(s->hcoef[k].data() + (size_t)r * d.cols, d.cols, codes.data() + (size_t)r * d.pitch, &rp[2 * r],
+                   &rp[2 * r + 1], 65535);
+    HIPCHK(s, hipMemcpy(d.p, codes.data(), codes.size() * 2, hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(d.rp, rp.data(), rp.size() * sizeof(float), hipMemcpyHostToDevice));
     return TW_OK;
   }
@@ -645,7 +657,7 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
     shape_of(c, coef_grid(k), &d.rows, &d.cols);
     d.pitch = tw::pitch_of(d.cols, pad);
     if ((e = dalloc(s, &d.p, (size_t)d.rows * d.pitch * ks))) return bail(e);
-    if (c->coef16 == 2 && (e = dalloc(s, &d.rp, (size_t)d.rows * 2 * sizeof(float)))) return bail(e);
+    if (c->coef16 >= 2 && (e = dalloc(s, &d.rp, (size_t)d.rows * 2 * sizeof(float)))) return bail(e);
     s->hcoef[k].assign((size_t)d.rows * d.cols, 0.0);
   }
   // characteristic SAT buffers

@@ -50,6 +50,7 @@
 #define TW_DISPATCH(p, F, ...)                                     \
   do {                                                             \
     if ((p).coef16 == 2) return F<bf16, c8, float>(__VA_ARGS__);   \
+    if ((p).coef16 == 3) return F<bf16, c16, float>(__VA_ARGS__);  \
     if ((p).coef16) return F<bf16, bf16, float>(__VA_ARGS__);      \
     return F<bf16, float, float>(__VA_ARGS__);                     \
   } while (0)
@@ -57,6 +58,7 @@
 #define TW_DISPATCH(p, F, ...)                                             \
   do {                                                                     \
     if ((p).coef16 == 2) return F<_Float16, c8, float>(__VA_ARGS__);       \
+    if ((p).coef16 == 3) return F<_Float16, c16, float>(__VA_ARGS__);      \
     if ((p).coef16) return F<_Float16, _Float16, float>(__VA_ARGS__);      \
     return F<_Float16, float, float>(__VA_ARGS__);                         \
   } while (0)
@@ -73,9 +75,16 @@ struct bf16 {
 struct c8 {
   uint8_t q;
 };
+// 16-bit material coefficient: the same with codes in [0, 65535]
+struct c16 {
+  uint16_t q;
+};
 struct c8f {
   float f;
 };
+// coefficient storage as codes of a per-row affine map
+template <class KT>
+constexpr bool is_code = std::is_same<KT, c8>::value || std::is_same<KT, c16>::value;
 
 template <class ST>
 struct St;
@@ -106,6 +115,11 @@ struct St<bf16> {
 };
 template <>
 struct St<c8> {
+  using C = float;
+  static __device__ __forceinline__ float ld(c8f v) { return v.f; }
+};
+template <>
+struct St<c16> {
   using C = float;
   static __device__ __forceinline__ float ld(c8f v) { return v.f; }
 };
@@ -208,9 +222,9 @@ template <class KT>
 struct K {
   const KT* p;
   int64_t pitch;
-  const float* rp;  // 8-bit codes: offset and step of each row (rp[2 r], rp[2 r + 1])
+  const float* rp;  // 8- or 16-bit codes: offset and step of each row (rp[2 r], rp[2 r + 1])
   __device__ __forceinline__ auto get(int ix, int iy) const {
-    if constexpr (std::is_same<KT, c8>::value)
+    if constexpr (is_code<KT>)
       return c8f{c8_dec(p[(int64_t)ix * pitch + iy].q, rp[2 * ix], rp[2 * ix + 1])};
     else
       return p[(int64_t)ix * pitch + iy];
@@ -1579,7 +1593,7 @@ template <class KT>
 __device__ __forceinline__ Q4<KT> kq4(const K<KT>& k, int row, int c) {
   Q4<KT> o;
   o.v = ld_v4s<KT>(k.p + (int64_t)row * k.pitch + c);
-  if constexpr (std::is_same<KT, c8>::value) {
+  if constexpr (is_code<KT>) {
     o.off = k.rp[2 * row];
     o.step = k.rp[2 * row + 1];
   }
@@ -1587,7 +1601,7 @@ __device__ __forceinline__ Q4<KT> kq4(const K<KT>& k, int row, int c) {
 }
 template <class KT, class CT>
 __device__ __forceinline__ void kcvt4(const Q4<KT>& k, CT (&o)[4]) {
-  if constexpr (std::is_same<KT, c8>::value) {
+  if constexpr (is_code<KT>) {
 #pragma unroll
     for (int j = 0; j < 4; j++) o[j] = c8_dec(k.v.v[j].q, k.off, k.step);
   } else {
@@ -2908,7 +2922,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(
 #if TW_PREC == 0
 bool fused_ok(const Plan& p) {
   return p.physics == TW_ELASTIC && !p.perx && !p.pery && p.energy_form == 0 && p.mx >= 12 && p.my >= 12 &&
-         p.coef16 != 2 &&
+         p.coef16 < 2 &&
          !(p.cpst == TW_CPST_3C || p.cpst == TW_CPST_M3C || p.cpst == TW_CPST_6C || is_k8(p.cpst));
 }
 
@@ -2984,7 +2998,7 @@ static hipError_t run_fused(const Plan& p, int what, const StepCtl& c, hipStream
 
 hipError_t TW_PUB(launch_fused)(const Plan& p, int what, const StepCtl& c, hipStream_t s) {
   // the fused step takes no 8-bit coefficients (fused_ok)
-  if (p.coef16 == 2) return hipErrorInvalidValue;
+  if (p.coef16 >= 2) return hipErrorInvalidValue;
 #if TW_PREC == 0
   return run_fused<double, double, double>(p, what, c, s);
 #elif TW_PREC == 1
