@@ -1719,6 +1719,7 @@ int batch_build(tw_batch* b, int nsteps) {
         return bfail(b, e, "batch: energy partial buffers: " + s->err);
       }
       s->part_cap = need;
+      s->gen++;  // graphs captured with the old buffers (other batches) are rebuilt
     }
   }
   std::vector<tw::Pending> carry(n);
