@@ -39,3 +39,13 @@ def twave_mod():
     import twave
     twave.lib()
     return twave
+
+
+@pytest.fixture
+def progress(capfd):
+    """Print a progress line past pytest's output capture: the long full-size
+    tests (minutes each) report every phase as it ends."""
+    def say(*msg):
+        with capfd.disabled():
+            print(*msg, flush=True)
+    return say

@@ -108,13 +108,15 @@ def source_off_step(cfg_dt, f0=15.0):
     return int(math.ceil(t_end / cfg_dt))
 
 
-def test_c2_elastic_fp32_full():
+def test_c2_elastic_fp32_full(progress):
     """4096^2, FP32, 2000 steps: determinism, energy after the source stops,
     FP32 vs FP64 within 1e-4 relative L2 (fields and receiver traces)."""
     m, nt = 4096, 2000
     print("case C2 full", m, nt, flush=True)
     a = run_full(m, T.FP32, "0", 0, nt)
+    progress("C2 first FP32 run done")
     b = run_full(m, T.FP32, "0", 0, nt)
+    progress("C2 second FP32 run done")
     fa, ra = state(a)
     fb, rb = state(b)
     for f in fa:
@@ -141,7 +143,7 @@ def test_c2_elastic_fp32_full():
     a.close()
 
 
-def test_c3_mixed_precision_full():
+def test_c3_mixed_precision_full(progress):
     """8192^2, 2000 steps (fields as stored value plus decoded residual).
     Against FP64 on the same quantized material: the K8 variant (BF16 + 8-bit
     residual codes) stays within its measured error and well below
@@ -160,6 +162,7 @@ def test_c3_mixed_precision_full():
     def full(precision, cpst, coef16, model_from=None):
         s = run_full(m, precision, cpst, coef16, nt, model_from=model_from)
         out = state(s)
+        progress("C3 run done: precision", precision, cpst, "coef16", coef16)
         return s, out
 
     k8, rk8 = full(T.BF16, "K8", 1)
@@ -278,7 +281,7 @@ def c4_problems(sizes, dx=0.01, vmax=4.5, nt=1600, seed=4, only=None):
     return out, src
 
 
-def test_c4_batch_as_configured():
+def test_c4_batch_as_configured(progress):
     """The batched-shots configuration as configured: 64 seeded problems with
     n uniform in [512, 2048], the tile height chosen by the autotuner, the
     1500-step loop as one graph.  The smallest and the largest problem equal
@@ -295,7 +298,9 @@ def test_c4_batch_as_configured():
     rows = B.autotune(20)
     assert rows in (4, 8, 16, 32, 64)
     assert all(s.current_step() == 0 for s in batched)
+    progress("C4 autotuned to %d rows" % rows)
     B.step(K)
+    progress("C4 batch of %d problems advanced %d steps" % (len(batched), K))
     for s in solo:
         s.step(K)
     for k, s in zip((lo, hi), solo):

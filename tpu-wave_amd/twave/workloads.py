@@ -191,6 +191,7 @@ class AdjointShots:
             if grad:
                 for a, b in zip(g, gi):
                     a += b
+                self._log("gradient shot %d/%d" % (i + 1, len(self.shots)))
         return total, g
 
     def close(self):
