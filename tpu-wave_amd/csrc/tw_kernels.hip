@@ -867,8 +867,8 @@ __global__ void __launch_bounds__(256) k_stress(Args<ST, KT, CT> A, Rects R) {
 
 // A band tile (256 points) run by one 64-lane block of a core launch: four
 // points per lane, in the tile order of map_point; the wave's energy sum goes
-// to the tile's partial.  No pending work here: block 0 of the launch is a
-// core block.
+// to the tile's partial.  No pending work here: the core tile (0, 0) of the
+// launch does it.
 __device__ __forceinline__ bool map_tile_point(const Rects& R, int b, int lin, int* ix, int* iy) {
   int k = 0;
   while (k + 1 < R.n && b >= R.cum[k + 1]) k++;
@@ -1973,13 +1973,17 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
 template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP, PH>)))
 k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C, int nbx, int ncore, Rects R) {
-  // blocks [0, ncore): core tiles; after them the band tiles of this half step
+  // blocks [0, nband): the band tiles of this half step, dispatched first so
+  // their latency-bound per-point work overlaps the core tiles; then the
+  // ncore core tiles
+  const int nband = (int)gridDim.x - ncore;
   const int L = blockIdx.x;
-  if (L >= ncore) {
-    band_tile<ST, KT, CT, 0>(*Sp, D, R, L - ncore);
+  if (L < nband) {
+    band_tile<ST, KT, CT, 0>(*Sp, D, R, L);
     return;
   }
-  vel3_body<ST, KT, CT, PH, CP>(Sp, D, C, L % nbx, L / nbx, nbx);
+  const int l = L - nband;
+  vel3_body<ST, KT, CT, PH, CP>(Sp, D, C, l % nbx, l / nbx, nbx);
 }
 
 // Stress half step, core region, streaming.
@@ -2339,13 +2343,17 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
 template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP, PH>)))
 k_str3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C, int nbx, int ncore, Rects R) {
-  // blocks [0, ncore): core tiles; after them the band tiles of this half step
+  // blocks [0, nband): the band tiles of this half step, dispatched first so
+  // their latency-bound per-point work overlaps the core tiles; then the
+  // ncore core tiles
+  const int nband = (int)gridDim.x - ncore;
   const int L = blockIdx.x;
-  if (L >= ncore) {
-    band_tile<ST, KT, CT, 1>(*Sp, D, R, L - ncore);
+  if (L < nband) {
+    band_tile<ST, KT, CT, 1>(*Sp, D, R, L);
     return;
   }
-  str3_body<ST, KT, CT, PH, CP>(Sp, D, C, L % nbx, L / nbx, nbx);
+  const int l = L - nband;
+  str3_body<ST, KT, CT, PH, CP>(Sp, D, C, l % nbx, l / nbx, nbx);
 }
 
 // ------------------------------------------------------------ dispatch
@@ -3034,16 +3042,20 @@ template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP, PH>)))
 k_vel3b(const BatchCore* __restrict__ items, int n, const BatchBand* __restrict__ bands, int nband,
         int core_blocks) {
-  // blocks [0, core_blocks): core tiles of all problems; after them their band tiles
-  if ((int)blockIdx.x >= core_blocks) {
-    const int bb = blockIdx.x - core_blocks;
+  // blocks [0, band_blocks): the band tiles of all problems, dispatched first
+  // (their per-point work overlaps the core tiles); then the core_blocks core
+  // tiles of all problems
+  const int band_blocks = (int)gridDim.x - core_blocks;
+  if ((int)blockIdx.x < band_blocks) {
+    const int bb = blockIdx.x;
     const BatchBand& E = bands[batch_item(&bands[0].b0, sizeof(BatchBand) / 4, nband, bb)];
     band_tile<ST, KT, CT, 0>(*(const Stat<ST, KT, CT>*)E.sp, E.d, E.r, bb - E.b0);
     return;
   }
-  const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, blockIdx.x);
+  const int cb = blockIdx.x - band_blocks;
+  const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, cb);
   const BatchCore& B = items[k];
-  const int l = blockIdx.x - B.b0;
+  const int l = cb - B.b0;
   vel3_body<ST, KT, CT, PH, CP>((const Stat<ST, KT, CT>*)B.sp, B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
 }
 
@@ -3051,16 +3063,20 @@ template <class ST, class KT, class CT, int PH, int CP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE<ST, CP, PH>)))
 k_str3b(const BatchCore* __restrict__ items, int n, const BatchBand* __restrict__ bands, int nband,
         int core_blocks) {
-  // blocks [0, core_blocks): core tiles of all problems; after them their band tiles
-  if ((int)blockIdx.x >= core_blocks) {
-    const int bb = blockIdx.x - core_blocks;
+  // blocks [0, band_blocks): the band tiles of all problems, dispatched first
+  // (their per-point work overlaps the core tiles); then the core_blocks core
+  // tiles of all problems
+  const int band_blocks = (int)gridDim.x - core_blocks;
+  if ((int)blockIdx.x < band_blocks) {
+    const int bb = blockIdx.x;
     const BatchBand& E = bands[batch_item(&bands[0].b0, sizeof(BatchBand) / 4, nband, bb)];
     band_tile<ST, KT, CT, 1>(*(const Stat<ST, KT, CT>*)E.sp, E.d, E.r, bb - E.b0);
     return;
   }
-  const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, blockIdx.x);
+  const int cb = blockIdx.x - band_blocks;
+  const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, cb);
   const BatchCore& B = items[k];
-  const int l = blockIdx.x - B.b0;
+  const int l = cb - B.b0;
   str3_body<ST, KT, CT, PH, CP>((const Stat<ST, KT, CT>*)B.sp, B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
 }
 
