@@ -63,12 +63,12 @@ struct Pending {
   int32_t rcv_it = 0;
 };
 
-// Decode of an 8-bit material coefficient: code q of a row whose affine map
-// has FP32 offset and step; two FP32 roundings (no fused multiply-add), the
-// same on the host and in the kernels.
+// Decode of an 8- or 16-bit material coefficient: code q of a row whose
+// affine map has FP32 offset and step, off + q * step with one rounding (a
+// fused multiply-add, correctly rounded on the host's libm and on the GPU
+// alike, so both sides decode to the same FP32 value).
 __host__ __device__ inline float c8_dec(unsigned q, float off, float step) {
-  const float t = (float)q * step;
-  return off + t;
+  return __builtin_fmaf((float)q, step, off);
 }
 
 // Row pitch in elements: columns rounded up to 128, so every row of every

@@ -1435,22 +1435,59 @@ __device__ __forceinline__ void upd(ST& s, ST& r, CT acc) {
   }
 }
 
-// 12 consecutive segment values around the lane's 4 columns: out[i] = column c - 4 + i
+// upd of the streaming bodies: with the 8-bit residual it also gives dq, the
+// decoded value of the code it writes (q * 2^shift of the new stored value,
+// exactly what k8_dec of the stored code returns), so the energy term reads
+// the new value without decoding that code again.  Same results as upd.
+template <class ST, class CT, int CP>
+__device__ __forceinline__ void upd_q(ST& s, ST& r, CT acc, CT& dq) {
+  if constexpr (sizeof(ST) == 2 && CPT<CP>::k8) {
+    using S = St<ST>;
+    const CT t = S::ld(s) + acc;
+    const ST hi = S::st(t);
+    s = hi;
+    const int sh = k8_shift(hi);
+    const float q = __builtin_rintf(__builtin_ldexpf(t - S::ld(hi), -sh));
+    const float cq = __builtin_fminf(__builtin_fmaxf(q, -127.f), 127.f);
+    r = __builtin_bit_cast(ST, (int16_t)(int)cq);
+    dq = __builtin_ldexpf(cq, sh);
+  } else {
+    upd<ST, CT, CP>(s, r, acc);
+    dq = (CT)0;
+  }
+}
+
+// the new value as the energy reads it (value_of of the stored pair), from
+// upd_q's dq
+template <class ST, class CT, int CP>
+__device__ __forceinline__ CT new_val(ST s, ST r, CT dq) {
+  if constexpr (sizeof(ST) == 2 && CPT<CP>::k8)
+    return (CT)St<ST>::ld(s) + dq;
+  else
+    return value_of<ST, CT>(s, r, CPT<CP>::km);
+}
+
+// Segment values around the lane's 4 columns: out[i] = column c - 4 + i for
+// i = 2..9, the columns c - 2 .. c + 5 that dy12 reads (out[0], out[1],
+// out[10], out[11] are left unset), as one 8-, one 16- and one 8-byte LDS read
 template <class CT>
 __device__ __forceinline__ void seg12(const CT* seg, int lane, CT out[12]) {
   const CT* p = seg + HALO + lane * VW - 4;
   if constexpr (sizeof(CT) == 4) {
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      const float4 v = *(const float4*)(p + 4 * q);
-      out[4 * q + 0] = v.x;
-      out[4 * q + 1] = v.y;
-      out[4 * q + 2] = v.z;
-      out[4 * q + 3] = v.w;
-    }
+    const float2 a = *(const float2*)(p + 2);
+    const float4 m = *(const float4*)(p + 4);
+    const float2 b = *(const float2*)(p + 8);
+    out[2] = a.x;
+    out[3] = a.y;
+    out[4] = m.x;
+    out[5] = m.y;
+    out[6] = m.z;
+    out[7] = m.w;
+    out[8] = b.x;
+    out[9] = b.y;
   } else {
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
+    for (int q = 1; q < 5; q++) {
       const double2 v = *(const double2*)(p + 2 * q);
       out[2 * q + 0] = v.x;
       out[2 * q + 1] = v.y;
@@ -1481,6 +1518,17 @@ __device__ __forceinline__ void src4(V4<ST>& sn, V4<ST>& rn, const Dyn& D, int g
     sn.v[q] = hit ? s1 : sn.v[q];
     rn.v[q] = hit ? r1 : rn.v[q];
   }
+}
+// the source on one value of the source row (the caller tests the row, which
+// is uniform; hit: this lane's column is the source column), keeping upd_q's
+// decoded residual in step with the code
+template <class ST, class CT, int CP>
+__device__ __forceinline__ void src1(ST& s, ST& r, CT& dq, double inc, bool hit) {
+  ST s1 = s, r1 = r;
+  add_source<ST, CT>(&s1, &r1, inc, CPT<CP>::km);
+  s = hit ? s1 : s;
+  r = hit ? r1 : r;
+  if constexpr (sizeof(ST) == 2 && CPT<CP>::k8) dq = hit ? (CT)k8_dec(r1, s1) : dq;
 }
 
 // MM energy of one cell (grid_energy.cpp:10-27, 180-201), core point (H = dx along both axes)
@@ -1707,8 +1755,12 @@ __device__ __forceinline__ double e_mm(CT lam, CT tmu, CT sxx, CT syy) {
 
 // Velocity half step, core region, streaming.
 template <class ST, class KT, class CT, int PH, int CP>
-__device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
+__device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& Dg, const Core& Cg, int bx,
                                           int by, int nbx) {
+  // the step's description and the core region in registers, read before any
+  // store of the launch
+  const Dyn D = Dg;
+  const Core C = Cg;
   constexpr bool RM = CPT<CP>::rm;
   constexpr int KM = CPT<CP>::km;
   constexpr bool RF = CPT<CP>::rf;
@@ -1716,15 +1768,20 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // syy or p row (y partner of vy)
   __shared__ double sh[8];
   const Stat<ST, KT, CT>& A = *Sp;
-  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   FR<ST> VX, VY, PX, PY, SXY;
   VX.load(A.f[TW_VX]);
   VY.load(A.f[TW_VY]);
   PX.load(A.f[PH == TW_ELASTIC ? TW_SXX : TW_P]);
   PY.load(A.f[PH == TW_ELASTIC ? TW_SYY : TW_P]);
   if constexpr (PH == TW_ELASTIC) SXY.load(A.f[TW_SXY]);
-  const K<KT> KBX = A.k[TW_C_BINV_VX];
-  const K<KT> KBY = A.k[TW_C_BINV_VY];
+  K<KT> KBX = A.k[TW_C_BINV_VX];
+  K<KT> KBY = A.k[TW_C_BINV_VY];
+  {  // every array has pitch_of(cols): one pitch for the M-column grids (NM,
+     // MM), one for the N-column grids (MN, NN), so row offsets are shared
+    const int64_t pm = VX.pitch, pn = VY.pitch;
+    PX.pitch = PY.pitch = KBX.pitch = pm;
+    SXY.pitch = KBY.pitch = pn;
+  }
   CT st[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) st[k] = A.t.st[k];
@@ -1742,6 +1799,10 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
   const bool inner = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
   const bool wave_full = eform == 0 && (inner || !pery);
   const int cmax = (my + 63) / 64 * 64 - 4;  // last 4-column chunk inside every row pitch
+  // block 0 finishes the previous launch's work after the solver description
+  // is read: no store precedes those loads, so they stay scalar loads also when
+  // the description comes from a batch table
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   double e = 0.0;
 
   auto rows = [&](auto fullc) {
@@ -1837,9 +1898,12 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
   // clamped rows), so a row costs one memory round trip; x windows rotate,
   // segment own values come from the windows, and energy terms are summed
   // unweighted and scaled by dx*dx once.
-  auto rows_full = [&]() {
-    bool in[4];
-    const bool full4 = col_mask(c, C.clo, C.chi, in);
+  // F4: every column of the strip is a core column (no masks, whole-vector
+  // stores, no energy selects)
+  auto rows_full = [&](auto f4c) {
+    constexpr bool F4 = decltype(f4c)::value;
+    bool in[4] = {true, true, true, true};
+    const bool full4 = F4 || col_mask(c, C.clo, C.chi, in);
     const int cl = min(c, cmax);  // load column: inside the allocation for every lane
     CT A0[4], A1[4], A2[4], A3[4];  // PX rows r-2..r+1 (x partner of vx)
     CT B0[4], B1[4], B2[4], B3[4];  // SXY rows r-1..r+2 (x partner of vy), elastic
@@ -1911,17 +1975,21 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         kcvt4<KT, CT>(kx, b);
         CT ya[12];
         if constexpr (PH == TW_ELASTIC) seg12<CT>(segA[buf], lane, ya);
+        // per value: update, source, energy term (the duplicated node row has weight 0)
+        const bool srow = D.src_on && D.src_grid == TW_GRID_NM && r == D.src_ix;
+        const bool erow = energy && !(perx && r == VX.rows - 1);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(qx.v[j], sx.v[j]);
           acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * b[j];
           if constexpr (PH == TW_ELASTIC) acc = acc + dy12<CT>(ya, j, 0, st) * b[j];
-          upd<ST, CT, CP>(sx.v[j], qx.v[j], acc);
-        }
-        src4<ST, CT, CP>(sx, qx, D, TW_GRID_NM, r, c);
-        if (energy && !(perx && r == VX.rows - 1)) {  // the duplicated node row has weight 0
-#pragma unroll
-          for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sx.v[j], qx.v[j], KM), b[j]) : 0.0;
+          CT dq;
+          upd_q<ST, CT, CP>(sx.v[j], qx.v[j], acc, dq);
+          if (srow) src1<ST, CT, CP>(sx.v[j], qx.v[j], dq, D.src_inc, D.src_iy - c == j);
+          if (erow) {
+            const double t = e_single<CT>(new_val<ST, CT, CP>(sx.v[j], qx.v[j], dq), b[j]);
+            ef += F4 || in[j] ? t : 0.0;
+          }
         }
         st_m<ST>(VX.row(r), c, sx, full4, in);
         if constexpr (RM) st_r4<ST, CP>(VX, r, c, qx, full4, in);
@@ -1931,17 +1999,19 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         kcvt4<KT, CT>(ky, b);
         CT yb[12];
         seg12<CT>(segB[buf], lane, yb);
+        const bool srow = D.src_on && D.src_grid == TW_GRID_MN && r == D.src_ix;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(qy.v[j], sy.v[j]);
           if constexpr (PH == TW_ELASTIC) acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * b[j];
           acc = acc + dy12<CT>(yb, j, 1, st) * b[j];
-          upd<ST, CT, CP>(sy.v[j], qy.v[j], acc);
-        }
-        src4<ST, CT, CP>(sy, qy, D, TW_GRID_MN, r, c);
-        if (energy) {
-#pragma unroll
-          for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sy.v[j], qy.v[j], KM), b[j]) : 0.0;
+          CT dq;
+          upd_q<ST, CT, CP>(sy.v[j], qy.v[j], acc, dq);
+          if (srow) src1<ST, CT, CP>(sy.v[j], qy.v[j], dq, D.src_inc, D.src_iy - c == j);
+          if (energy) {
+            const double t = e_single<CT>(new_val<ST, CT, CP>(sy.v[j], qy.v[j], dq), b[j]);
+            ef += F4 || in[j] ? t : 0.0;
+          }
         }
         st_m<ST>(VY.row(r), c, sy, full4, in);
         if constexpr (RM) st_r4<ST, CP>(VY, r, c, qy, full4, in);
@@ -1960,10 +2030,14 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
     }
     e = ef * (dx * dx);
   };
-  if (wave_full)
-    rows_full();
-  else
+  if (wave_full) {
+    if (base >= C.clo && base + STRIP <= C.chi)
+      rows_full(std::true_type{});
+    else
+      rows_full(std::false_type{});
+  } else {
     rows(std::false_type{});
+  }
   if (energy) {
     const double bsum = block_sum(e, sh);
     if (lane == 0) D.part[by * nbx + bx] = bsum;
@@ -1988,8 +2062,12 @@ k_vel3(const Stat<ST, KT, CT>* __restrict__ Sp, Dyn D, Core C, int nbx, int ncor
 
 // Stress half step, core region, streaming.
 template <class ST, class KT, class CT, int PH, int CP>
-__device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& D, const Core& C, int bx,
+__device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ Sp, const Dyn& Dg, const Core& Cg, int bx,
                                           int by, int nbx) {
+  // the step's description and the core region in registers, read before any
+  // store of the launch
+  const Dyn D = Dg;
+  const Core C = Cg;
   constexpr bool RM = CPT<CP>::rm;
   constexpr int KM = CPT<CP>::km;
   constexpr bool RF = CPT<CP>::rf;
@@ -1997,7 +2075,6 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // vy row (y partner of MM / p)
   __shared__ double sh[8];
   const Stat<ST, KT, CT>& A = *Sp;
-  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   FR<ST> VX, VY, XN, X0, X1;
   VX.load(A.f[TW_VX]);
   VY.load(A.f[TW_VY]);
@@ -2006,9 +2083,15 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
     X1.load(A.f[TW_SYY]);
     XN.load(A.f[TW_SXY]);
   }
-  const K<KT> K0 = A.k[PH == TW_ELASTIC ? TW_C_LAMBDA : TW_C_KAPPA];
-  const K<KT> K1 = A.k[TW_C_TWOMU];
-  const K<KT> KMU = A.k[TW_C_MU];
+  K<KT> K0 = A.k[PH == TW_ELASTIC ? TW_C_LAMBDA : TW_C_KAPPA];
+  K<KT> K1 = A.k[TW_C_TWOMU];
+  K<KT> KMU = A.k[TW_C_MU];
+  {  // one pitch for the M-column grids (NM, MM), one for the N-column grids
+     // (MN, NN): every array has pitch_of(cols)
+    const int64_t pm = VX.pitch, pn = VY.pitch;
+    X0.pitch = X1.pitch = K0.pitch = K1.pitch = pm;
+    XN.pitch = KMU.pitch = pn;
+  }
   CT st[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) st[k] = A.t.st[k];
@@ -2026,6 +2109,10 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
   const bool inner = base >= HALO && base + STRIP + HALO <= my && base >= C.clo && base + STRIP <= C.chi;
   const bool wave_full = eform == 0 && (inner || !pery);
   const int cmax = (my + 63) / 64 * 64 - 4;  // last 4-column chunk inside every row pitch
+  // block 0 finishes the previous launch's work after the solver description
+  // is read: no store precedes those loads, so they stay scalar loads also when
+  // the description comes from a batch table
+  if (bx == 0 && by == 0) do_pending<ST, KT, CT>(A, D, sh);
   double e = 0.0;
 
   auto rows = [&](auto fullc) {
@@ -2161,9 +2248,10 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
   // FULL strips (energy form 0): as in k_vel3, all loads of a row first,
   // rotating windows, segment own values from the windows, unweighted
   // energy terms scaled by dx*dx once.
-  auto rows_full = [&]() {
-    bool in[4];
-    const bool full4 = col_mask(c, C.clo, C.chi, in);
+  auto rows_full = [&](auto f4c) {
+    constexpr bool F4 = decltype(f4c)::value;  // every column a core column
+    bool in[4] = {true, true, true, true};
+    const bool full4 = F4 || col_mask(c, C.clo, C.chi, in);
     const int cl = min(c, cmax);  // load column: inside the allocation for every lane
     CT A0[4], A1[4], A2[4], A3[4];  // VY rows r-2..r+1 (x partner of sxy), elastic
     CT B0[4], B1[4], B2[4], B3[4];  // VX rows r-1..r+2 (x partner of MM)
@@ -2245,17 +2333,21 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         kcvt4<KT, CT>(kn, mu);
         CT ya[12];
         seg12<CT>(segA[buf], lane, ya);
+        // per value: update, source, energy term (the duplicated node row has weight 0)
+        const bool srow = D.src_on && D.src_grid == TW_GRID_NN && r == D.src_ix;
+        const bool erow = energy && !(perx && r == XN.rows - 1);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(qn.v[j], sn.v[j]);
           acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * mu[j];
           acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
-          upd<ST, CT, CP>(sn.v[j], qn.v[j], acc);
-        }
-        src4<ST, CT, CP>(sn, qn, D, TW_GRID_NN, r, c);
-        if (energy && !(perx && r == XN.rows - 1)) {  // the duplicated node row has weight 0
-#pragma unroll
-          for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(sn.v[j], qn.v[j], KM), mu[j]) : 0.0;
+          CT dq;
+          upd_q<ST, CT, CP>(sn.v[j], qn.v[j], acc, dq);
+          if (srow) src1<ST, CT, CP>(sn.v[j], qn.v[j], dq, D.src_inc, D.src_iy - c == j);
+          if (erow) {
+            const double t = e_single<CT>(new_val<ST, CT, CP>(sn.v[j], qn.v[j], dq), mu[j]);
+            ef += F4 || in[j] ? t : 0.0;
+          }
         }
         st_m<ST>(XN.row(r), c, sn, full4, in);
         if constexpr (RM) st_r4<ST, CP>(XN, r, c, qn, full4, in);
@@ -2267,6 +2359,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           CT lam[4], tmu[4];
           kcvt4<KT, CT>(k0, lam);
           kcvt4<KT, CT>(k1, tmu);
+          const bool srow = D.src_on && D.src_grid == TW_GRID_MM && r == D.src_ix;
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             CT a0 = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
@@ -2279,17 +2372,19 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
             a0 = a0 + w * lam[j];
             a1 = a1 + w * lam[j];
             a1 = a1 + w * tmu[j];
-            upd<ST, CT, CP>(s0.v[j], q0.v[j], a0);
-            upd<ST, CT, CP>(s1.v[j], q1.v[j], a1);
-          }
-          src4<ST, CT, CP>(s0, q0, D, TW_GRID_MM, r, c);
-          src4<ST, CT, CP>(s1, q1, D, TW_GRID_MM, r, c);
-          if (energy) {
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-              ef += in[j] ? e_mm<CT>(lam[j], tmu[j], value_of<ST, CT>(s0.v[j], q0.v[j], KM),
-                                     value_of<ST, CT>(s1.v[j], q1.v[j], KM))
-                          : 0.0;
+            CT d0, d1;
+            upd_q<ST, CT, CP>(s0.v[j], q0.v[j], a0, d0);
+            upd_q<ST, CT, CP>(s1.v[j], q1.v[j], a1, d1);
+            if (srow) {
+              const bool hit = D.src_iy - c == j;
+              src1<ST, CT, CP>(s0.v[j], q0.v[j], d0, D.src_inc, hit);
+              src1<ST, CT, CP>(s1.v[j], q1.v[j], d1, D.src_inc, hit);
+            }
+            if (energy) {
+              const double t = e_mm<CT>(lam[j], tmu[j], new_val<ST, CT, CP>(s0.v[j], q0.v[j], d0),
+                                        new_val<ST, CT, CP>(s1.v[j], q1.v[j], d1));
+              ef += F4 || in[j] ? t : 0.0;
+            }
           }
           st_m<ST>(X0.row(r), c, s0, full4, in);
           st_m<ST>(X1.row(r), c, s1, full4, in);
@@ -2300,17 +2395,19 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         } else {  // ---- p (MM, acoustic)
           CT kap[4];
           kcvt4<KT, CT>(k0, kap);
+          const bool srow = D.src_on && D.src_grid == TW_GRID_MM && r == D.src_ix;
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             CT acc = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
             acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * kap[j];
             acc = acc + dy12<CT>(yb, j, 0, st) * kap[j];
-            upd<ST, CT, CP>(s0.v[j], q0.v[j], acc);
-          }
-          src4<ST, CT, CP>(s0, q0, D, TW_GRID_MM, r, c);
-          if (energy) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) ef += in[j] ? e_single<CT>(value_of<ST, CT>(s0.v[j], q0.v[j], KM), kap[j]) : 0.0;
+            CT dq;
+            upd_q<ST, CT, CP>(s0.v[j], q0.v[j], acc, dq);
+            if (srow) src1<ST, CT, CP>(s0.v[j], q0.v[j], dq, D.src_inc, D.src_iy - c == j);
+            if (energy) {
+              const double t = e_single<CT>(new_val<ST, CT, CP>(s0.v[j], q0.v[j], dq), kap[j]);
+              ef += F4 || in[j] ? t : 0.0;
+            }
           }
           st_m<ST>(X0.row(r), c, s0, full4, in);
           if constexpr (RM) st_r4<ST, CP>(X0, r, c, q0, full4, in);
@@ -2330,10 +2427,14 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
     }
     e = ef * (dx * dx);
   };
-  if (wave_full)
-    rows_full();
-  else
+  if (wave_full) {
+    if (base >= C.clo && base + STRIP <= C.chi)
+      rows_full(std::true_type{});
+    else
+      rows_full(std::false_type{});
+  } else {
     rows(std::false_type{});
+  }
   if (energy) {
     const double bsum = block_sum(e, sh);
     if (lane == 0) D.part[by * nbx + bx] = bsum;
@@ -3026,6 +3127,21 @@ hipError_t TW_PUB(launch_fused)(const Plan& p, int what, const StepCtl& c, hipSt
 // block finds its problem by a binary search over the first-block offsets,
 // then runs the same body as the single-problem kernels on that problem's
 // device Stat with the step's Dyn.
+
+// A problem's device Stat as a global-memory pointer: the table holds it as a
+// plain pointer, and through a generic one the body's reads of the solver
+// description would be flat (vector) loads; through a global one they are
+// scalar loads, as in the single-problem kernels.
+template <class ST, class KT, class CT>
+__device__ __forceinline__ const Stat<ST, KT, CT>* stat_of(const void* p) {
+  typedef const __attribute__((address_space(1))) Stat<ST, KT, CT>* G;
+  const uint64_t u = (uint64_t)p;  // the same for every lane of the block
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (const Stat<ST, KT, CT>*)(G)(((uint64_t)hi << 32) | lo);
+}
+
+// index of the problem whose block range holds `block`
 __device__ __forceinline__ int batch_item(const int32_t* b0, int stride_words, int n, int block) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
@@ -3049,14 +3165,14 @@ k_vel3b(const BatchCore* __restrict__ items, int n, const BatchBand* __restrict_
   if ((int)blockIdx.x < band_blocks) {
     const int bb = blockIdx.x;
     const BatchBand& E = bands[batch_item(&bands[0].b0, sizeof(BatchBand) / 4, nband, bb)];
-    band_tile<ST, KT, CT, 0>(*(const Stat<ST, KT, CT>*)E.sp, E.d, E.r, bb - E.b0);
+    band_tile<ST, KT, CT, 0>(*stat_of<ST, KT, CT>(E.sp), E.d, E.r, bb - E.b0);
     return;
   }
   const int cb = blockIdx.x - band_blocks;
   const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, cb);
   const BatchCore& B = items[k];
   const int l = cb - B.b0;
-  vel3_body<ST, KT, CT, PH, CP>((const Stat<ST, KT, CT>*)B.sp, B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
+  vel3_body<ST, KT, CT, PH, CP>(stat_of<ST, KT, CT>(B.sp), B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
 }
 
 template <class ST, class KT, class CT, int PH, int CP>
@@ -3070,14 +3186,14 @@ k_str3b(const BatchCore* __restrict__ items, int n, const BatchBand* __restrict_
   if ((int)blockIdx.x < band_blocks) {
     const int bb = blockIdx.x;
     const BatchBand& E = bands[batch_item(&bands[0].b0, sizeof(BatchBand) / 4, nband, bb)];
-    band_tile<ST, KT, CT, 1>(*(const Stat<ST, KT, CT>*)E.sp, E.d, E.r, bb - E.b0);
+    band_tile<ST, KT, CT, 1>(*stat_of<ST, KT, CT>(E.sp), E.d, E.r, bb - E.b0);
     return;
   }
   const int cb = blockIdx.x - band_blocks;
   const int k = batch_item(&items[0].b0, sizeof(BatchCore) / 4, n, cb);
   const BatchCore& B = items[k];
   const int l = cb - B.b0;
-  str3_body<ST, KT, CT, PH, CP>((const Stat<ST, KT, CT>*)B.sp, B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
+  str3_body<ST, KT, CT, PH, CP>(stat_of<ST, KT, CT>(B.sp), B.d, B.c, l % B.nbx, l / B.nbx, B.nbx);
 }
 
 template <class ST, class KT, class CT>
@@ -3085,7 +3201,7 @@ __global__ void __launch_bounds__(256) k_bandb(const BatchBand* __restrict__ ite
   __shared__ double sh[8];
   const int k = batch_item(&items[0].b0, sizeof(BatchBand) / 4, n, blockIdx.x);
   const BatchBand& B = items[k];
-  const Stat<ST, KT, CT>& A = *(const Stat<ST, KT, CT>*)B.sp;
+  const Stat<ST, KT, CT>& A = *stat_of<ST, KT, CT>(B.sp);
   if (half == 0)
     vel_pt<ST, KT, CT>(A, B.d, B.r, blockIdx.x - B.b0, 0, sh);
   else
