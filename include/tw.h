@@ -381,7 +381,7 @@ int tw_profile_end(tw_solver* s, double* ms_velocity, double* ms_stress, int32_t
  * benchmark reports beside the solver's.  No counterpart in the reference. */
 int tw_stream_copy(void* dst, const void* src, size_t bytes, void* hip_stream);
 
-/* ---- discrete adjoint (acoustic; FP64 or FP32 storage; no compensation) ----
+/* ---- discrete adjoint (acoustic and elastic; FP64 or FP32 storage; no compensation) ----
  *
  * Gradient of a receiver misfit L(traces) with respect to the material
  * coefficients, by the transposed time step run backwards with sqrt(N)-step
@@ -402,6 +402,16 @@ int tw_adjoint_forward(tw_solver* s, int32_t nsteps, int32_t interval);
  * forward run. */
 int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, double* grad_binv_vx,
                    double* grad_binv_vy);
+
+/* tw_adjoint_run_coefs: the same for either physics, with grads[k] the
+ * output for coefficient id k (TW_C_*, six entries, null skipped; non-null
+ * only for coefficients the physics uses): acoustic kappa and 1/rho on NM and
+ * MN; elastic lambda and 2 mu (MM), mu (NN), 1/rho on NM and MN.  Gradients
+ * through the characteristic impedances (sqrt(modulus / b) of the normal and,
+ * elastic, the tangential velocity) are included.  Like tw_adjoint_run, it
+ * differentiates the reference's misfit path (cuda/src/cpp/forward.cpp:414-445)
+ * with respect to density and moduli; the reference has no adjoint. */
+int tw_adjoint_run_coefs(tw_solver* s, const double* rcv_adjoint, double* const* grads);
 
 /* ---- batched shots ----
  *

@@ -103,9 +103,12 @@ def test_adjoint_fp32_close_to_fp64():
 
 
 def test_adjoint_rejects_unsupported():
-    s, _ = C.make_pair(physics="elastic", mx=16, my=16, bc="FFPP", precision="fp32", nt=4)
+    s, _ = C.make_pair(physics="acoustic", mx=16, my=16, bc="FFPP", precision="bf16", nt=4)
     with pytest.raises(T.TwError):
         s.adjoint_forward(2)
+    s1, _ = C.make_pair(physics="acoustic", mx=16, my=16, bc="FFPP", precision="fp32", cpst="6R", nt=4)
+    with pytest.raises(T.TwError):
+        s1.adjoint_forward(2)
     s2 = acoustic_solver("PPPP")
     with pytest.raises(T.TwError):
         s2.adjoint_run(np.zeros((s2.nrcv, 2)))  # no checkpointed run yet

@@ -1346,7 +1346,27 @@ int tw_check_finite_noalloc(tw_solver* s) {
 // ------------------------------------------------------------ discrete adjoint
 namespace {
 
-const int kAdjFields[3] = {TW_VX, TW_VY, TW_P};
+// the fields of the adjoint state and checkpoints, and the coefficients with a gradient
+std::vector<int> adj_fields(const tw_solver* s) {
+  if (s->cfg.physics == TW_ELASTIC) return {TW_VX, TW_VY, TW_SXX, TW_SYY, TW_SXY};
+  return {TW_VX, TW_VY, TW_P};
+}
+std::vector<int> adj_coefs(const tw_solver* s) {
+  if (s->cfg.physics == TW_ELASTIC) return {TW_C_BINV_VX, TW_C_BINV_VY, TW_C_MU, TW_C_LAMBDA, TW_C_TWOMU};
+  return {TW_C_KAPPA, TW_C_BINV_VX, TW_C_BINV_VY};
+}
+// the field whose grid (and pitch) a coefficient shares
+int coef_field(int k) {
+  switch (k) {
+    case TW_C_BINV_VX: return TW_VX;
+    case TW_C_BINV_VY: return TW_VY;
+    case TW_C_MU: return TW_SXY;
+    case TW_C_KAPPA: return TW_P;
+    default: return TW_SXX;
+  }
+}
+// line length of side `side`'s boundary points of velocity f (across the axis)
+int side_len(const tw::Plan& p, int side, int f) { return side < 2 ? p.f[f].cols : p.f[f].rows; }
 
 // forward taps along an axis of M cells (the per-point kernels' ddx/ddy):
 // out_t 1 = N output (M + 1 points, partner M), 0 = M output (partner N)
@@ -1423,7 +1443,6 @@ int upload_transpose(tw_solver* s, const tw::Tables& t, int M, bool periodic, in
 
 int adj_check(tw_solver* s) {
   const tw_config* c = &s->cfg;
-  if (c->physics != TW_ACOUSTIC) return fail(&s->err, TW_EINVAL, "adjoint: acoustic physics only");
   if (c->precision != TW_FP64 && c->precision != TW_FP32) return fail(&s->err, TW_EINVAL, "adjoint: FP64 or FP32 storage only");
   if (c->cpst != TW_CPST_0) return fail(&s->err, TW_EINVAL, "adjoint: no compensation (cpst 0) only");
   return TW_OK;
@@ -1434,25 +1453,31 @@ int adj_setup(tw_solver* s) {
   Plan& p = s->plan;
   const size_t as = arith_size(s->cfg.precision);
   int e;
-  for (int f : kAdjFields) {
+  for (int f : adj_fields(s)) {
     const tw::DevField& d = p.f[f];
     if ((e = dalloc(s, &s->adj.mu[f], (size_t)d.rows * d.pitch * as))) return e;
   }
-  const int gf[3] = {TW_P, TW_VX, TW_VY};
-  for (int i = 0; i < 3; i++) {
-    const tw::DevField& d = p.f[gf[i]];
-    if ((e = dalloc(s, (void**)&s->adj.g[i], (size_t)d.rows * d.pitch * 8))) return e;
+  for (int k : adj_coefs(s)) {
+    const tw::DevField& d = p.f[coef_field(k)];
+    if ((e = dalloc(s, (void**)&s->adj.g[k], (size_t)d.rows * d.pitch * 8))) return e;
   }
-  for (int side = 0; side < 4; side++) {  // normal velocity of each side: vx on x sides, vy on y sides
-    const tw::DevField& d = p.f[side < 2 ? TW_VX : TW_VY];
-    if ((e = dalloc(s, (void**)&s->adj.gz[side], (size_t)(side < 2 ? d.cols : d.rows) * 8))) return e;
+  const bool el = s->cfg.physics == TW_ELASTIC;
+  for (int side = 0; side < 4; side++) {
+    // normal velocity of each side (vx on x sides, vy on y sides); elastic:
+    // the tangential one too (vy on x sides, vx on y sides)
+    const int fn = side < 2 ? TW_VX : TW_VY, ft = side < 2 ? TW_VY : TW_VX;
+    if ((e = dalloc(s, (void**)&s->adj.gz[side], (size_t)side_len(p, side, fn) * 8))) return e;
+    if (!el) continue;
+    if ((e = dalloc(s, (void**)&s->adj.gzt[side], (size_t)side_len(p, side, ft) * 8))) return e;
+    if ((e = dalloc(s, &s->adj.dn[side], (size_t)side_len(p, side, fn) * as))) return e;
+    if ((e = dalloc(s, &s->adj.dtg[side], (size_t)side_len(p, side, ft) * 3 * as))) return e;
   }
   if ((e = upload_transpose(s, p.t, p.mx, p.perx, 1, &s->adj.gt[0]))) return e;
   if ((e = upload_transpose(s, p.t, p.my, p.pery, 1, &s->adj.gt[1]))) return e;
   if ((e = upload_transpose(s, p.t, p.mx, p.perx, 0, &s->adj.dt[0]))) return e;
   if ((e = upload_transpose(s, p.t, p.my, p.pery, 0, &s->adj.dt[1]))) return e;
   s->adj_state_bytes = 0;
-  for (int f : kAdjFields) s->adj_state_bytes += (size_t)p.f[f].rows * p.f[f].pitch * state_size(s->cfg.precision);
+  for (int f : adj_fields(s)) s->adj_state_bytes += (size_t)p.f[f].rows * p.f[f].pitch * state_size(s->cfg.precision);
   s->adj_taps = true;
   return TW_OK;
 }
@@ -1462,7 +1487,7 @@ int state_copy(tw_solver* s, void* slot, bool save) {
   Plan& p = s->plan;
   char* q = (char*)slot;
   const size_t es = state_size(s->cfg.precision);
-  for (int f : kAdjFields) {
+  for (int f : adj_fields(s)) {
     const size_t b = (size_t)p.f[f].rows * p.f[f].pitch * es;
     if (save)
       HIPCHK(s, hipMemcpyAsync(q, p.f[f].s, b, hipMemcpyDeviceToDevice, s->stream));
@@ -1477,7 +1502,7 @@ tw::StateView state_view(tw_solver* s, void* slot) {
   tw::StateView v;
   char* q = (char*)slot;
   const size_t es = state_size(s->cfg.precision);
-  for (int f : kAdjFields) {
+  for (int f : adj_fields(s)) {
     v.s[f] = q;
     q += (size_t)s->plan.f[f].rows * s->plan.f[f].pitch * es;
   }
@@ -1523,14 +1548,64 @@ int tw_adjoint_forward(tw_solver* s, int32_t nsteps, int32_t interval) {
   return TW_OK;
 }
 
-int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, double* grad_binv_vx,
-                   double* grad_binv_vy) {
-  if (!s || !rcv_adjoint) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
-  if (!s->adj_done) return fail(&s->err, TW_ESTATE, "tw_adjoint_run needs a tw_adjoint_forward run first");
+namespace {
+
+// z = sqrt(modulus / b) at a characteristic side's boundary point (the
+// impedance upload_z sets): dz/dmodulus = z / (2 modulus), dz/db = -z / (2 b).
+// Adds g_z dz/d(coefficient) to the coefficient gradients for the normal
+// velocity (modulus kappa, or lambda + 2 mu of the adjacent MM cell) and, in
+// the elastic case, the tangential one (mu of the adjacent NN node).
+void impedance_chain(tw_solver* s, const std::vector<double>* gz, const std::vector<double>* gzt,
+                     std::vector<double>* g) {
+  const tw_config* c = &s->cfg;
+  const bool el = c->physics == TW_ELASTIC;
+  const int Mx = c->mx, My = c->my;
+  for (int side = 0; side < 4; side++) {
+    if (c->bc[side] != 'C') continue;
+    const int axis = side / 2, right = side % 2;
+    for (int f = TW_VX; f <= TW_VY; f++) {
+      const bool normal = (axis == 0) == (f == TW_VX);
+      const std::vector<double>& gs = normal ? gz[side] : gzt[side];
+      if (gs.empty()) continue;
+      const tw::DevField& F = s->plan.f[f];
+      const int kb = f == TW_VX ? TW_C_BINV_VX : TW_C_BINV_VY;
+      const std::vector<double>& b = s->hcoef[kb];
+      for (int j = 0; j < (int)gs.size(); j++) {
+        const int a = axis == 0 ? (right ? F.rows - 1 : 0) : (right ? F.cols - 1 : 0);
+        const size_t qv = axis == 0 ? (size_t)a * F.cols + j : (size_t)j * F.cols + a;
+        if (normal) {
+          const int am = axis == 0 ? (right ? Mx - 1 : 0) : (right ? My - 1 : 0);
+          const size_t qm = axis == 0 ? (size_t)am * My + j : (size_t)j * My + am;
+          const double mod = el ? s->hcoef[TW_C_LAMBDA][qm] + s->hcoef[TW_C_TWOMU][qm] : s->hcoef[TW_C_KAPPA][qm];
+          const double z = std::sqrt(mod / b[qv]), d = gs[j] * z / (2 * mod);
+          if (el) {
+            g[TW_C_LAMBDA][qm] += d;
+            g[TW_C_TWOMU][qm] += d;
+          } else {
+            g[TW_C_KAPPA][qm] += d;
+          }
+          g[kb][qv] -= gs[j] * z / (2 * b[qv]);
+        } else {
+          const int an = axis == 0 ? (right ? Mx : 0) : (right ? My : 0);
+          const size_t qn = axis == 0 ? (size_t)an * (My + 1) + j : (size_t)j * (My + 1) + an;
+          const double mu = s->hcoef[TW_C_MU][qn];
+          const double z = std::sqrt(mu / b[qv]);
+          g[TW_C_MU][qn] += gs[j] * z / (2 * mu);
+          g[kb][qv] -= gs[j] * z / (2 * b[qv]);
+        }
+      }
+    }
+  }
+}
+
+// the backward sweep of tw_adjoint_run / tw_adjoint_run_coefs: out[k] for
+// coefficient id k (null: skipped)
+int adjoint_sweep(tw_solver* s, const double* rcv_adjoint, double* const* out) {
   (void)hipSetDevice(s->cfg.device);
   Plan& p = s->plan;
   const int n = s->adj_n, I = s->adj_I, it0 = s->adj_it0;
   const size_t sb = s->adj_state_bytes;
+  const bool el = s->cfg.physics == TW_ELASTIC;
   int e;
   // receiver sources
   const size_t ninj = (size_t)(s->nrcv > 0 ? s->nrcv : 1) * n;
@@ -1544,13 +1619,16 @@ int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, 
   s->adj.inj = s->adj_inj;
   s->adj.nt = n;
   const size_t as = arith_size(s->cfg.precision);
-  for (int f : kAdjFields) HIPCHK(s, hipMemsetAsync(s->adj.mu[f], 0, (size_t)p.f[f].rows * p.f[f].pitch * as, s->stream));
-  const int gf[3] = {TW_P, TW_VX, TW_VY};
-  for (int i = 0; i < 3; i++)
-    HIPCHK(s, hipMemsetAsync(s->adj.g[i], 0, (size_t)p.f[gf[i]].rows * p.f[gf[i]].pitch * 8, s->stream));
+  for (int f : adj_fields(s))
+    HIPCHK(s, hipMemsetAsync(s->adj.mu[f], 0, (size_t)p.f[f].rows * p.f[f].pitch * as, s->stream));
+  for (int k : adj_coefs(s)) {
+    const tw::DevField& d = p.f[coef_field(k)];
+    HIPCHK(s, hipMemsetAsync(s->adj.g[k], 0, (size_t)d.rows * d.pitch * 8, s->stream));
+  }
   for (int side = 0; side < 4; side++) {
-    const tw::DevField& d = p.f[side < 2 ? TW_VX : TW_VY];
-    HIPCHK(s, hipMemsetAsync(s->adj.gz[side], 0, (size_t)(side < 2 ? d.cols : d.rows) * 8, s->stream));
+    HIPCHK(s, hipMemsetAsync(s->adj.gz[side], 0, (size_t)side_len(p, side, side < 2 ? TW_VX : TW_VY) * 8, s->stream));
+    if (el)
+      HIPCHK(s, hipMemsetAsync(s->adj.gzt[side], 0, (size_t)side_len(p, side, side < 2 ? TW_VY : TW_VX) * 8, s->stream));
   }
   for (int k = s->adj_nckpt - 1; k >= 0; k--) {
     const int b = k * I, en = std::min(n, b + I);
@@ -1573,47 +1651,58 @@ int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, 
   if ((e = state_copy(s, s->adj_final, false))) return e;
   s->it = it0 + n;
   s->ep_valid = false;
-  std::vector<double> g[3];
-  for (int i = 0; i < 3; i++) {
-    const tw::DevField& d = p.f[gf[i]];
-    g[i].resize((size_t)d.rows * d.cols);
-    HIPCHK(s, hipMemcpy2DAsync(g[i].data(), (size_t)d.cols * 8, s->adj.g[i], (size_t)d.pitch * 8, (size_t)d.cols * 8,
+  std::vector<double> g[6];
+  for (int k : adj_coefs(s)) {
+    const tw::DevField& d = p.f[coef_field(k)];
+    g[k].resize((size_t)d.rows * d.cols);
+    HIPCHK(s, hipMemcpy2DAsync(g[k].data(), (size_t)d.cols * 8, s->adj.g[k], (size_t)d.pitch * 8, (size_t)d.cols * 8,
                                d.rows, hipMemcpyDeviceToHost, s->stream));
   }
-  std::vector<double> gz[4];
+  std::vector<double> gz[4], gzt[4];
   for (int side = 0; side < 4; side++) {
     if (s->cfg.bc[side] != 'C') continue;
-    const tw::DevField& d = p.f[side < 2 ? TW_VX : TW_VY];
-    gz[side].resize(side < 2 ? d.cols : d.rows);
+    gz[side].resize(side_len(p, side, side < 2 ? TW_VX : TW_VY));
     HIPCHK(s, hipMemcpyAsync(gz[side].data(), s->adj.gz[side], gz[side].size() * 8, hipMemcpyDeviceToHost, s->stream));
+    if (!el) continue;
+    gzt[side].resize(side_len(p, side, side < 2 ? TW_VY : TW_VX));
+    HIPCHK(s, hipMemcpyAsync(gzt[side].data(), s->adj.gzt[side], gzt[side].size() * 8, hipMemcpyDeviceToHost,
+                             s->stream));
   }
   HIPCHK(s, hipStreamSynchronize(s->stream));
-  // the impedance z = sqrt(kappa / b) of the normal velocity on a
-  // characteristic side (kappa of the adjacent cell): dz/dkappa = z / (2 kappa),
-  // dz/db = -z / (2 b)
-  const int Mx = s->cfg.mx, My = s->cfg.my;
-  const std::vector<double>& kap = s->hcoef[TW_C_KAPPA];
-  for (int side = 0; side < 4; side++) {
-    if (gz[side].empty()) continue;
-    const bool xs = side < 2, right = side % 2;
-    const int f = xs ? TW_VX : TW_VY;
-    const tw::DevField& d = p.f[f];
-    const std::vector<double>& b = s->hcoef[xs ? TW_C_BINV_VX : TW_C_BINV_VY];
-    std::vector<double>& gb = g[xs ? 1 : 2];
-    for (int j = 0; j < (int)gz[side].size(); j++) {
-      const int a = xs ? (right ? d.rows - 1 : 0) : (right ? d.cols - 1 : 0);
-      const size_t qv = xs ? (size_t)a * d.cols + j : (size_t)j * d.cols + a;
-      const int am = xs ? (right ? Mx - 1 : 0) : (right ? My - 1 : 0);
-      const size_t qm = xs ? (size_t)am * My + j : (size_t)j * My + am;
-      const double z = std::sqrt(kap[qm] / b[qv]);
-      g[0][qm] += gz[side][j] * z / (2 * kap[qm]);
-      gb[qv] -= gz[side][j] * z / (2 * b[qv]);
-    }
-  }
-  double* outs[3] = {grad_kappa, grad_binv_vx, grad_binv_vy};
-  for (int i = 0; i < 3; i++)
-    if (outs[i]) memcpy(outs[i], g[i].data(), g[i].size() * 8);
+  impedance_chain(s, gz, gzt, g);
+  for (int k : adj_coefs(s))
+    if (out[k]) memcpy(out[k], g[k].data(), g[k].size() * 8);
   return TW_OK;
+}
+
+int adjoint_ready(tw_solver* s, const double* rcv_adjoint) {
+  if (!s || !rcv_adjoint) return fail(s ? &s->err : nullptr, TW_EINVAL, "null argument");
+  if (!s->adj_done) return fail(&s->err, TW_ESTATE, "tw_adjoint_run needs a tw_adjoint_forward run first");
+  return TW_OK;
+}
+
+}  // namespace
+
+int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, double* grad_binv_vx,
+                   double* grad_binv_vy) {
+  int e = adjoint_ready(s, rcv_adjoint);
+  if (e) return e;
+  if (s->cfg.physics != TW_ACOUSTIC) return fail(&s->err, TW_EINVAL, "tw_adjoint_run: acoustic; use tw_adjoint_run_coefs");
+  double* out[6] = {};
+  out[TW_C_KAPPA] = grad_kappa;
+  out[TW_C_BINV_VX] = grad_binv_vx;
+  out[TW_C_BINV_VY] = grad_binv_vy;
+  return adjoint_sweep(s, rcv_adjoint, out);
+}
+
+int tw_adjoint_run_coefs(tw_solver* s, const double* rcv_adjoint, double* const* grads) {
+  int e = adjoint_ready(s, rcv_adjoint);
+  if (e) return e;
+  if (!grads) return fail(&s->err, TW_EINVAL, "null argument");
+  for (int k = 0; k < 6; k++)
+    if (grads[k] && !coef_in(s->cfg.physics, k))
+      return fail(&s->err, TW_EINVAL, "gradient of a coefficient this physics does not use");
+  return adjoint_sweep(s, rcv_adjoint, grads);
 }
 
 // ------------------------------------------------------------ batched shots

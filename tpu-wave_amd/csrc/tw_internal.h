@@ -139,7 +139,7 @@ struct StepCtl {
   Pending pend;
 };
 
-// Discrete adjoint of the acoustic step (device buffers owned by the solver).
+// Discrete adjoint of the time step (device buffers owned by the solver).
 // Transposed tap lists along an axis: for a partner index j, the outputs i
 // whose stencil reads j and the coefficient (arithmetic type, dt/dx included).
 constexpr int TMAX = 8;
@@ -149,15 +149,21 @@ struct TapList {
   void* cf = nullptr;      // [len][TMAX] coefficients
 };
 struct AdjPlan {
-  void* mu[NF] = {};        // adjoint of VX, VY, P (arithmetic type, field pitch)
-  double* g[3] = {};        // d misfit / d coefficient: kappa (MM), 1/rho on NM, 1/rho on MN (field pitch)
+  void* mu[NF] = {};        // adjoint of the fields (arithmetic type, field pitch)
+  double* g[6] = {};        // d misfit / d coefficient, by coefficient id (TW_C_*), on its grid (field pitch)
   double* gz[4] = {};       // d misfit / d impedance of the normal velocity along each characteristic side
+  double* gzt[4] = {};      // elastic: the same for the tangential velocity
+  // elastic: adjoint increments of the characteristic self-terms, computed
+  // from the incoming adjoint before any is applied: one per line of the
+  // normal velocity's boundary points, three per line of the tangential one's
+  void* dn[4] = {};
+  void* dtg[4] = {};
   TapList gt[2];            // transpose of the N-output derivative of M data (x, y)
   TapList dt[2];            // transpose of the M-output derivative of N data (x, y)
   const double* inj = nullptr;  // receiver adjoint sources [nrcv][nt]
   int32_t nt = 0;
 };
-// forward fields of a saved state (VX, VY, P storage arrays, field pitch)
+// forward fields of a saved state (the physics' storage arrays, field pitch)
 struct StateView {
   void* s[NF] = {};
 };

@@ -3305,8 +3305,10 @@ struct TapT {
 template <class CT>
 struct AdjArgs {
   CT* mu[NF];
-  double* g[3];
-  double* gz[4];
+  double* g[6];  // by coefficient id
+  double* gz[4], *gzt[4];
+  CT* dn[4];
+  CT* dtg[4];
   TapT<CT> gt[2], dt[2];
   const double* inj;
   int32_t nt, it_rel;
@@ -3329,7 +3331,7 @@ __global__ void __launch_bounds__(256) k_adj1(Args<ST, KT, CT> U, AdjArgs<CT> J)
   if (ix < P.rows && iy < P.cols) {  // g_kappa
     const CT d = ddx<ST, CT>(U.f[TW_VX], 0, ix, iy, P.rows, U.perx, U.mx, U.t) +
                  ddy<ST, CT>(U.f[TW_VY], 0, ix, iy, P.cols, U.pery, U.my, U.t);
-    J.g[0][(int64_t)ix * pp + iy] += (double)J.mu[TW_P][(int64_t)ix * pp + iy] * (double)d;
+    J.g[TW_C_KAPPA][(int64_t)ix * pp + iy] += (double)J.mu[TW_P][(int64_t)ix * pp + iy] * (double)d;
   }
   const F<ST>& X = U.f[TW_VX];
   if (ix < X.rows && iy < X.cols) {  // mu_vx += Dx^T (kappa mu_p), along x
@@ -3368,7 +3370,7 @@ __global__ void __launch_bounds__(256) k_adj2(Args<ST, KT, CT> U, AdjArgs<CT> J)
     CT t = ddx<ST, CT>(P, 1, ix, iy, X.rows, U.perx, U.mx, U.t);
     if (!U.perx) t = sat_add(U, t, TW_VX, TW_P, 0, 1, ix, iy, u, (CT)1);
     const double m = (double)J.mu[TW_VX][q];
-    J.g[1][q] += m * (double)t;
+    J.g[TW_C_BINV_VX][q] += m * (double)t;
     for (int side = 0; side < 2; side++)  // increment -a z u b: d/dz = -a u b
       if (!U.perx && U.bc[side] == 2 && ix == (side == 0 ? 0 : X.rows - 1))
         J.gz[side][iy] += m * (double)(-((side == 0 ? U.t.ainvN_L : U.t.ainvN_R) * u) * kv(U, TW_C_BINV_VX, ix, iy));
@@ -3379,7 +3381,7 @@ __global__ void __launch_bounds__(256) k_adj2(Args<ST, KT, CT> U, AdjArgs<CT> J)
     CT t = ddy<ST, CT>(P, 1, ix, iy, Y.cols, U.pery, U.my, U.t);
     if (!U.pery) t = sat_add(U, t, TW_VY, TW_P, 1, 1, ix, iy, u, (CT)1);
     const double m = (double)J.mu[TW_VY][q];
-    J.g[2][q] += m * (double)t;
+    J.g[TW_C_BINV_VY][q] += m * (double)t;
     for (int side = 0; side < 2; side++)
       if (!U.pery && U.bc[2 + side] == 2 && iy == (side == 0 ? 0 : Y.cols - 1))
         J.gz[2 + side][ix] += m * (double)(-((side == 0 ? U.t.ainvN_L : U.t.ainvN_R) * u) * kv(U, TW_C_BINV_VY, ix, iy));
@@ -3448,16 +3450,276 @@ __global__ void __launch_bounds__(256) k_adj3(Args<ST, KT, CT> U, AdjArgs<CT> J)
 
 // receiver sources of forward step it_rel (relative to the adjoint run); one
 // thread, since receivers may share a point
+struct Pitches {
+  int64_t p[NF];
+};
 template <class CT>
-__global__ void k_adj_inject(AdjArgs<CT> J, int64_t p_vx, int64_t p_vy, int64_t p_p) {
+__global__ void k_adj_inject(AdjArgs<CT> J, Pitches P) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   for (int g = 0; g < 2; g++)
     for (int r = 0; r < J.nrcv[g]; r++) {
       const Receiver rc = J.rcv[g][r];
-      const int64_t pitch = rc.field == TW_VX ? p_vx : (rc.field == TW_VY ? p_vy : p_p);
-      CT& m = J.mu[rc.field][(int64_t)rc.ix * pitch + rc.iy];
+      CT& m = J.mu[rc.field][(int64_t)rc.ix * P.p[rc.field] + rc.iy];
       m = m + (CT)J.inj[(int64_t)rc.row * J.nt + J.it_rel];
     }
+}
+
+// ---- elastic
+// One elastic leapfrog step (no compensation) is Step = S o V with
+//   V: vx <- vx + bx (Dx_N sxx + Dy_M sxy + SAT),  vy <- vy + by (Dx_M sxy + Dy_N syy + SAT)
+//   S: sxx <- sxx + (lam + 2mu) Dx_M vx + lam Dy_M vy
+//      syy <- syy + lam Dx_M vx + (lam + 2mu) Dy_M vy
+//      sxy <- sxy + mu (Dx_N vy + Dy_N vx)
+// (D*_N: N-output derivative of M data, D*_M: M-output derivative of N data;
+// SAT terms act on the velocity points only.)  A characteristic side adds to
+// its normal velocity's boundary point -ai z b u (u its own value) and to each
+// of its tangential velocity's 3 boundary points -ai_w z b_w uhat, uhat the
+// 3-point extrapolation of those points.  Step^T = V^T o S^T:
+//   E1 (velocities after the step): g_lam, g_2mu, g_mu; mu_v += D^T(C mu_s)
+//   E2 (state before the step): g_bx, g_by; mu_s += G^T(b mu_v) + SAT sends^T
+//   E3 (state before the step, one thread per boundary line): the
+//      characteristic self-terms' increments from the incoming mu_v, and the
+//      impedance sensitivities; E4 and E5 add the increments of the normal
+//      and of the tangential boundary points (two launches: the corners of a
+//      field are boundary points of both kinds).
+
+// E1
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_eadj1(Args<ST, KT, CT> U, AdjArgs<CT> J) {
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = blockIdx.y * blockDim.y + threadIdx.y;
+  const F<ST>& X = U.f[TW_VX];
+  const F<ST>& Y = U.f[TW_VY];
+  const F<ST>& M = U.f[TW_SXX];
+  const F<ST>& N = U.f[TW_SXY];
+  const CT* m0 = J.mu[TW_SXX];
+  const CT* m1 = J.mu[TW_SYY];
+  const CT* mn = J.mu[TW_SXY];
+  if (ix < M.rows && iy < M.cols) {  // g_lam, g_2mu (MM)
+    const double dvx = (double)ddx<ST, CT>(X, 0, ix, iy, M.rows, U.perx, U.mx, U.t);
+    const double dvy = (double)ddy<ST, CT>(Y, 0, ix, iy, M.cols, U.pery, U.my, U.t);
+    const int64_t q = M.at(ix, iy);
+    const double a = (double)m0[q], b = (double)m1[q];
+    J.g[TW_C_LAMBDA][q] += (a + b) * (dvx + dvy);
+    J.g[TW_C_TWOMU][q] += a * dvx + b * dvy;
+  }
+  if (ix < N.rows && iy < N.cols) {  // g_mu (NN)
+    const double d = (double)ddx<ST, CT>(Y, 1, ix, iy, N.rows, U.perx, U.mx, U.t) +
+                     (double)ddy<ST, CT>(X, 1, ix, iy, N.cols, U.pery, U.my, U.t);
+    const int64_t q = N.at(ix, iy);
+    J.g[TW_C_MU][q] += (double)mn[q] * d;
+  }
+  if (ix < X.rows && iy < X.cols) {  // mu_vx += Dx_M^T (lam (mu_sxx + mu_syy) + 2mu mu_sxx) + Dy_N^T (mu mu_sxy)
+    CT acc = (CT)0;
+    const TapT<CT>& L = J.dt[0];
+    for (int e = 0; e < L.n[ix]; e++) {
+      const int i = L.idx[ix * TMAX + e];
+      const int64_t q = M.at(i, iy);
+      acc = acc + L.cf[ix * TMAX + e] *
+                      (kv(U, TW_C_LAMBDA, i, iy) * (m0[q] + m1[q]) + kv(U, TW_C_TWOMU, i, iy) * m0[q]);
+    }
+    const TapT<CT>& G = J.gt[1];
+    for (int e = 0; e < G.n[iy]; e++) {
+      const int j = G.idx[iy * TMAX + e];
+      acc = acc + G.cf[iy * TMAX + e] * (kv(U, TW_C_MU, ix, j) * mn[N.at(ix, j)]);
+    }
+    J.mu[TW_VX][X.at(ix, iy)] += acc;
+  }
+  if (ix < Y.rows && iy < Y.cols) {  // mu_vy += Dy_M^T (lam (mu_sxx + mu_syy) + 2mu mu_syy) + Dx_N^T (mu mu_sxy)
+    CT acc = (CT)0;
+    const TapT<CT>& L = J.dt[1];
+    for (int e = 0; e < L.n[iy]; e++) {
+      const int j = L.idx[iy * TMAX + e];
+      const int64_t q = M.at(ix, j);
+      acc = acc + L.cf[iy * TMAX + e] *
+                      (kv(U, TW_C_LAMBDA, ix, j) * (m0[q] + m1[q]) + kv(U, TW_C_TWOMU, ix, j) * m1[q]);
+    }
+    const TapT<CT>& G = J.gt[0];
+    for (int e = 0; e < G.n[ix]; e++) {
+      const int i = G.idx[ix * TMAX + e];
+      acc = acc + G.cf[ix * TMAX + e] * (kv(U, TW_C_MU, i, iy) * mn[N.at(i, iy)]);
+    }
+    J.mu[TW_VY][Y.at(ix, iy)] += acc;
+  }
+}
+
+// E2
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_eadj2(Args<ST, KT, CT> U, AdjArgs<CT> J) {
+  using S = St<ST>;
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = blockIdx.y * blockDim.y + threadIdx.y;
+  const F<ST>& X = U.f[TW_VX];
+  const F<ST>& Y = U.f[TW_VY];
+  const F<ST>& M = U.f[TW_SXX];
+  const F<ST>& N = U.f[TW_SXY];
+  const CT* ux = J.mu[TW_VX];
+  const CT* uy = J.mu[TW_VY];
+  if (ix < X.rows && iy < X.cols) {  // g_bx: the bracket of the vx update (uhat of this state in U.uhat)
+    const int64_t q = X.at(ix, iy);
+    const CT u = (CT)S::ld(X.s[q]);
+    CT t = ddx<ST, CT>(M, 1, ix, iy, X.rows, U.perx, U.mx, U.t);
+    if (!U.perx) t = sat_add(U, t, TW_VX, TW_SXX, 0, 1, ix, iy, u, (CT)1);
+    t = t + ddy<ST, CT>(N, 0, ix, iy, X.cols, U.pery, U.my, U.t);
+    if (!U.pery) t = sat_add(U, t, TW_VX, TW_SXY, 1, 0, ix, iy, u, (CT)1);
+    J.g[TW_C_BINV_VX][q] += (double)ux[q] * (double)t;
+  }
+  if (ix < Y.rows && iy < Y.cols) {  // g_by
+    const int64_t q = Y.at(ix, iy);
+    const CT u = (CT)S::ld(Y.s[q]);
+    CT t = ddx<ST, CT>(N, 0, ix, iy, Y.rows, U.perx, U.mx, U.t);
+    if (!U.perx) t = sat_add(U, t, TW_VY, TW_SXY, 0, 0, ix, iy, u, (CT)1);
+    t = t + ddy<ST, CT>(U.f[TW_SYY], 1, ix, iy, Y.cols, U.pery, U.my, U.t);
+    if (!U.pery) t = sat_add(U, t, TW_VY, TW_SYY, 1, 1, ix, iy, u, (CT)1);
+    J.g[TW_C_BINV_VY][q] += (double)uy[q] * (double)t;
+  }
+  if (ix < M.rows && iy < M.cols) {
+    // mu_sxx += Dx_N^T (bx mu_vx) + the sends of the vx boundary rows (3-point projection)
+    CT acc = (CT)0;
+    const TapT<CT>& G = J.gt[0];
+    for (int e = 0; e < G.n[ix]; e++) {
+      const int j = G.idx[ix * TMAX + e];
+      acc = acc + G.cf[ix * TMAX + e] * (kv(U, TW_C_BINV_VX, j, iy) * ux[X.at(j, iy)]);
+    }
+    if (!U.perx) {
+      if (U.bc[0] && ix < 3)
+        acc = acc + (U.t.ainvN_L * U.t.projM_L[ix]) * (kv(U, TW_C_BINV_VX, 0, iy) * ux[X.at(0, iy)]);
+      const int w = ix - (M.rows - 3);
+      if (U.bc[1] && w >= 0)
+        acc = acc - (U.t.ainvN_R * U.t.projM_R[w]) *
+                        (kv(U, TW_C_BINV_VX, X.rows - 1, iy) * ux[X.at(X.rows - 1, iy)]);
+    }
+    J.mu[TW_SXX][M.at(ix, iy)] += acc;
+    // mu_syy += Dy_N^T (by mu_vy) + the sends of the vy boundary columns
+    acc = (CT)0;
+    const TapT<CT>& H = J.gt[1];
+    for (int e = 0; e < H.n[iy]; e++) {
+      const int j = H.idx[iy * TMAX + e];
+      acc = acc + H.cf[iy * TMAX + e] * (kv(U, TW_C_BINV_VY, ix, j) * uy[Y.at(ix, j)]);
+    }
+    if (!U.pery) {
+      if (U.bc[2] && iy < 3)
+        acc = acc + (U.t.ainvN_L * U.t.projM_L[iy]) * (kv(U, TW_C_BINV_VY, ix, 0) * uy[Y.at(ix, 0)]);
+      const int w = iy - (M.cols - 3);
+      if (U.bc[3] && w >= 0)
+        acc = acc - (U.t.ainvN_R * U.t.projM_R[w]) *
+                        (kv(U, TW_C_BINV_VY, ix, Y.cols - 1) * uy[Y.at(ix, Y.cols - 1)]);
+    }
+    J.mu[TW_SYY][M.at(ix, iy)] += acc;
+  }
+  if (ix < N.rows && iy < N.cols) {
+    // mu_sxy += Dy_M^T (bx mu_vx) + Dx_M^T (by mu_vy) + the sends read by the
+    // tangential velocities' 3 boundary points of each side
+    CT acc = (CT)0;
+    const TapT<CT>& L1 = J.dt[1];
+    for (int e = 0; e < L1.n[iy]; e++) {
+      const int j = L1.idx[iy * TMAX + e];
+      acc = acc + L1.cf[iy * TMAX + e] * (kv(U, TW_C_BINV_VX, ix, j) * ux[X.at(ix, j)]);
+    }
+    const TapT<CT>& L0 = J.dt[0];
+    for (int e = 0; e < L0.n[ix]; e++) {
+      const int i = L0.idx[ix * TMAX + e];
+      acc = acc + L0.cf[ix * TMAX + e] * (kv(U, TW_C_BINV_VY, i, iy) * uy[Y.at(i, iy)]);
+    }
+    if (!U.pery && ix < X.rows) {
+      if (U.bc[2] && iy == 0)
+        for (int w = 0; w < 3; w++) acc = acc + U.t.ainvM_L[w] * (kv(U, TW_C_BINV_VX, ix, w) * ux[X.at(ix, w)]);
+      if (U.bc[3] && iy == N.cols - 1)
+        for (int w = 0; w < 3; w++) {
+          const int c = X.cols - 3 + w;
+          acc = acc - U.t.ainvM_R[w] * (kv(U, TW_C_BINV_VX, ix, c) * ux[X.at(ix, c)]);
+        }
+    }
+    if (!U.perx && iy < Y.cols) {
+      if (U.bc[0] && ix == 0)
+        for (int w = 0; w < 3; w++) acc = acc + U.t.ainvM_L[w] * (kv(U, TW_C_BINV_VY, w, iy) * uy[Y.at(w, iy)]);
+      if (U.bc[1] && ix == N.rows - 1)
+        for (int w = 0; w < 3; w++) {
+          const int r = Y.rows - 3 + w;
+          acc = acc - U.t.ainvM_R[w] * (kv(U, TW_C_BINV_VY, r, iy) * uy[Y.at(r, iy)]);
+        }
+    }
+    J.mu[TW_SXY][N.at(ix, iy)] += acc;
+  }
+}
+
+// E3: per characteristic side and boundary line t, from the incoming adjoint
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_eadj3(Args<ST, KT, CT> U, AdjArgs<CT> J) {
+  using S = St<ST>;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int side = 0; side < 4; side++) {
+    if (U.bc[side] != 2) continue;
+    const int axis = side >> 1, right = side & 1;
+    {  // normal velocity: vx on the x sides, vy on the y sides (N-type along the axis)
+      const int f = axis == 0 ? TW_VX : TW_VY;
+      const F<ST>& V = U.f[f];
+      if (t < (axis == 0 ? V.cols : V.rows)) {
+        const int a = right ? (axis == 0 ? V.rows : V.cols) - 1 : 0;
+        const int ix = axis == 0 ? a : t, iy = axis == 0 ? t : a;
+        const int64_t q = V.at(ix, iy);
+        const CT ai = right ? U.t.ainvN_R : U.t.ainvN_L;
+        const CT b = kv(U, f == TW_VX ? TW_C_BINV_VX : TW_C_BINV_VY, ix, iy);
+        const CT m = J.mu[f][q];
+        const CT u = (CT)S::ld(V.s[q]);
+        J.dn[side][t] = (-(ai * U.z[f == TW_VX ? 0 : 1][side][t]) * b) * m;
+        J.gz[side][t] += (double)m * (double)(-(ai * u) * b);
+      }
+    }
+    {  // tangential velocity: vy on the x sides, vx on the y sides (M-type along the axis)
+      const int f = axis == 0 ? TW_VY : TW_VX;
+      const F<ST>& V = U.f[f];
+      const int n_along = axis == 0 ? V.rows : V.cols;
+      if (t < (axis == 0 ? V.cols : V.rows)) {
+        const CT* pm = right ? U.t.projM_R : U.t.projM_L;
+        const CT* aiw = right ? U.t.ainvM_R : U.t.ainvM_L;
+        const CT z = U.z[f == TW_VX ? 0 : 1][side][t];
+        CT uh = (CT)0;  // as k_uhat
+        for (int w = 0; w < 3; w++) {
+          const int a = right ? n_along - 3 + w : w;
+          uh = uh + pm[w] * (CT)S::ld(V.s[axis == 0 ? V.at(a, t) : V.at(t, a)]);
+        }
+        CT sum = (CT)0;
+        double gs = 0.0;
+        for (int w = 0; w < 3; w++) {
+          const int a = right ? n_along - 3 + w : w;
+          const int ix = axis == 0 ? a : t, iy = axis == 0 ? t : a;
+          const CT b = kv(U, f == TW_VX ? TW_C_BINV_VX : TW_C_BINV_VY, ix, iy);
+          const CT m = J.mu[f][V.at(ix, iy)];
+          sum = sum + (-(aiw[w] * z) * b) * m;
+          gs += (double)m * (double)(-(aiw[w] * uh) * b);
+        }
+        for (int w = 0; w < 3; w++) J.dtg[side][3 * t + w] = pm[w] * sum;
+        J.gzt[side][t] += gs;
+      }
+    }
+  }
+}
+
+// E4: the normal velocities' boundary points; E5 (tangential = 1): the
+// tangential velocities' 3 boundary points of each line
+template <class ST, class KT, class CT>
+__global__ void __launch_bounds__(256) k_eadj45(Args<ST, KT, CT> U, AdjArgs<CT> J, int tangential) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int side = 0; side < 4; side++) {
+    if (U.bc[side] != 2) continue;
+    const int axis = side >> 1, right = side & 1;
+    const int f = (axis == 0) == !tangential ? TW_VX : TW_VY;
+    const F<ST>& V = U.f[f];
+    if (t >= (axis == 0 ? V.cols : V.rows)) continue;
+    const int n_along = axis == 0 ? V.rows : V.cols;
+    if (!tangential) {
+      const int a = right ? n_along - 1 : 0;
+      CT& m = J.mu[f][axis == 0 ? V.at(a, t) : V.at(t, a)];
+      m = m + J.dn[side][t];
+    } else {
+      for (int w = 0; w < 3; w++) {
+        const int a = right ? n_along - 3 + w : w;
+        CT& m = J.mu[f][axis == 0 ? V.at(a, t) : V.at(t, a)];
+        m = m + J.dtg[side][3 * t + w];
+      }
+    }
+  }
 }
 
 template <class ST, class KT, class CT>
@@ -3465,8 +3727,13 @@ static hipError_t run_adjoint_step(const Plan& p, const AdjPlan& a, const StateV
                                    int it, int it_rel, hipStream_t s) {
   AdjArgs<CT> J;
   for (int f = 0; f < NF; f++) J.mu[f] = (CT*)a.mu[f];
-  for (int i = 0; i < 3; i++) J.g[i] = a.g[i];
-  for (int i = 0; i < 4; i++) J.gz[i] = a.gz[i];
+  for (int i = 0; i < 6; i++) J.g[i] = a.g[i];
+  for (int i = 0; i < 4; i++) {
+    J.gz[i] = a.gz[i];
+    J.gzt[i] = a.gzt[i];
+    J.dn[i] = (CT*)a.dn[i];
+    J.dtg[i] = (CT*)a.dtg[i];
+  }
   for (int d = 0; d < 2; d++) {
     J.gt[d] = TapT<CT>{a.gt[d].n, a.gt[d].idx, (const CT*)a.gt[d].cf};
     J.dt[d] = TapT<CT>{a.dt[d].n, a.dt[d].idx, (const CT*)a.dt[d].cf};
@@ -3486,13 +3753,29 @@ static hipError_t run_adjoint_step(const Plan& p, const AdjPlan& a, const StateV
     if (cur.s[f]) U1.f[f].s = (ST*)cur.s[f];
     if (prev.s[f]) U0.f[f].s = (ST*)prev.s[f];
   }
-  hipLaunchKernelGGL((k_adj_inject<CT>), dim3(1), dim3(64), 0, s, J, p.f[TW_VX].pitch, p.f[TW_VY].pitch,
-                     p.f[TW_P].pitch);
+  Pitches P;
+  for (int f = 0; f < NF; f++) P.p[f] = p.f[f].pitch;
+  hipLaunchKernelGGL((k_adj_inject<CT>), dim3(1), dim3(64), 0, s, J, P);
   const dim3 b(64, 4), g((unsigned)((p.my + 1 + 63) / 64), (unsigned)((p.mx + 1 + 3) / 4));
-  hipLaunchKernelGGL((k_adj1<ST, KT, CT>), g, b, 0, s, U1, J);
-  hipLaunchKernelGGL((k_adj2<ST, KT, CT>), g, b, 0, s, U0, J);
   const int n = (p.mx > p.my ? p.mx : p.my) + 1;
-  hipLaunchKernelGGL((k_adj3<ST, KT, CT>), dim3((n + 255) / 256), dim3(256), 0, s, U0, J);
+  const dim3 gl((unsigned)((n + 255) / 256));
+  if (p.physics == TW_ACOUSTIC) {
+    hipLaunchKernelGGL((k_adj1<ST, KT, CT>), g, b, 0, s, U1, J);
+    hipLaunchKernelGGL((k_adj2<ST, KT, CT>), g, b, 0, s, U0, J);
+    hipLaunchKernelGGL((k_adj3<ST, KT, CT>), gl, dim3(256), 0, s, U0, J);
+    return hipGetLastError();
+  }
+  const bool chr = p.bc[0] == 'C' || p.bc[1] == 'C' || p.bc[2] == 'C' || p.bc[3] == 'C';
+  hipLaunchKernelGGL((k_eadj1<ST, KT, CT>), g, b, 0, s, U1, J);
+  // the tangential velocities' boundary extrapolation of the state before the
+  // step, which the SAT terms of E2 read (the forward step recomputes it)
+  if (chr) hipLaunchKernelGGL((k_uhat<ST, KT, CT>), gl, dim3(64, 4), 0, s, U0);
+  hipLaunchKernelGGL((k_eadj2<ST, KT, CT>), g, b, 0, s, U0, J);
+  if (chr) {
+    hipLaunchKernelGGL((k_eadj3<ST, KT, CT>), gl, dim3(256), 0, s, U0, J);
+    hipLaunchKernelGGL((k_eadj45<ST, KT, CT>), gl, dim3(256), 0, s, U0, J, 0);
+    hipLaunchKernelGGL((k_eadj45<ST, KT, CT>), gl, dim3(256), 0, s, U0, J, 1);
+  }
   return hipGetLastError();
 }
 

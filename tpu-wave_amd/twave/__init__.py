@@ -157,6 +157,7 @@ _SIGS = [
     ("tw_profile_end", ctypes.c_int, [_vp, _dp, _dp, _ip]),
     ("tw_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     ("tw_adjoint_run", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    ("tw_adjoint_run_coefs", ctypes.c_int, [_vp, _dp, ctypes.POINTER(_dp)]),
     ("tw_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(_vp)]),
     ("tw_batch_destroy", ctypes.c_int, [_vp]),
     ("tw_batch_last_error", ctypes.c_char_p, [_vp]),
@@ -403,8 +404,8 @@ class Solver:
         self._chk(lib().tw_set_stream(self.h, ctypes.c_void_p(stream_ptr)))
 
     def adjoint_forward(self, nsteps, interval=0):
-        """Advance nsteps keeping sqrt(N) checkpoints for adjoint_run
-        (acoustic, FP64/FP32 storage, cpst 0)."""
+        """Advance nsteps keeping sqrt(N) checkpoints for adjoint_run /
+        adjoint_run_coefs (FP64/FP32 storage, cpst 0)."""
         self._chk(lib().tw_adjoint_forward(self.h, int(nsteps), int(interval)))
         self._adj_steps = int(nsteps)
 
@@ -420,6 +421,22 @@ class Solver:
         gy = np.zeros(self.get_coef(C_BINV_VY).shape)
         self._chk(lib().tw_adjoint_run(self.h, _d(w), _d(gk), _d(gx), _d(gy)))
         return gk, gx, gy
+
+    def adjoint_run_coefs(self, rcv_adjoint):
+        """dL/d(coefficient) for every coefficient of the physics
+        (tw_adjoint_run_coefs): {coefficient id: gradient on its grid}; elastic
+        C_LAMBDA, C_TWOMU (MM), C_MU (NN), C_BINV_VX (NM), C_BINV_VY (MN)."""
+        w = np.ascontiguousarray(rcv_adjoint, dtype=np.float64)
+        if self._adj_steps and (w.ndim != 2 or w.shape[0] != self.nrcv or w.shape[1] != self._adj_steps):
+            raise ValueError("rcv_adjoint shape %s, expected (%d, %d)" % (w.shape, self.nrcv, self._adj_steps))
+        ks = (C_BINV_VX, C_BINV_VY, C_MU, C_LAMBDA, C_TWOMU) if self.cfg.physics == ELASTIC else \
+            (C_KAPPA, C_BINV_VX, C_BINV_VY)
+        out = {k: np.zeros(self.get_coef(k).shape) for k in ks}
+        ptrs = (_dp * 6)()
+        for k, g in out.items():
+            ptrs[k] = _d(g)
+        self._chk(lib().tw_adjoint_run_coefs(self.h, _d(w), ptrs))
+        return out
 
     def check_finite(self):
         e = lib().tw_check_finite(self.h)

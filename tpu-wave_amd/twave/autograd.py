@@ -1,8 +1,10 @@
-"""torch.autograd.Function over the acoustic solver: receiver traces as a
-differentiable function of the material coefficients (kappa on MM, 1/rho on
-NM and MN, or kappa and rho itself).  Forward = checkpointed forward run
-(tw_adjoint_forward), backward = the hand-written discrete adjoint
-(tw_adjoint_run) with the incoming trace gradient as the receiver sources.
+"""torch.autograd.Function over the solver: receiver traces as a
+differentiable function of the material coefficients.  Acoustic: kappa on MM,
+1/rho on NM and MN (or kappa and rho itself).  Elastic: lambda and 2 mu on
+MM, mu on NN, 1/rho on NM and MN (or lambda, mu and rho).  Forward =
+checkpointed forward run (tw_adjoint_forward), backward = the hand-written
+discrete adjoint (tw_adjoint_run_coefs) with the incoming trace gradient as
+the receiver sources.
 
 The solver keeps its own configuration, source and receivers; each forward
 starts from the state captured when the operator was built.  The C-ABI takes
@@ -18,29 +20,33 @@ import itertools
 import numpy as np
 import torch
 
-from . import C_BINV_VX, C_BINV_VY, C_KAPPA
+from . import C_BINV_VX, C_BINV_VY, C_KAPPA, C_LAMBDA, C_MU, C_TWOMU
 
 _ids = itertools.count(1)
+
+ACOUSTIC_COEFS = (C_KAPPA, C_BINV_VX, C_BINV_VY)
+ELASTIC_COEFS = (C_LAMBDA, C_TWOMU, C_MU, C_BINV_VX, C_BINV_VY)
 
 
 def _run_forward(op, coefs):
     op.solver.restore(op.state0)
-    for k, v in zip((C_KAPPA, C_BINV_VX, C_BINV_VY), coefs):
+    for k, v in zip(op.coefs, coefs):
         op.solver.set_coef(k, v)
     op.solver.adjoint_forward(op.nsteps, op.interval)
 
 
 class _Traces(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, kappa, binv_vx, binv_vy, op):
-        coefs = [t.detach().to("cpu", torch.float64).numpy().copy() for t in (kappa, binv_vx, binv_vy)]
+    def forward(ctx, op, *tensors):
+        coefs = [t.detach().to("cpu", torch.float64).numpy().copy() for t in tensors]
         _run_forward(op, coefs)
         fid = next(_ids)
         op.last_forward = fid
         tr = op.solver.receivers()[:, -op.nsteps:]
         ctx.op, ctx.fid, ctx.coefs = op, fid, coefs
-        ctx.like = kappa
-        return torch.from_numpy(np.ascontiguousarray(tr)).to(device=kappa.device, dtype=kappa.dtype)
+        like = tensors[0]
+        ctx.like = like
+        return torch.from_numpy(np.ascontiguousarray(tr)).to(device=like.device, dtype=like.dtype)
 
     @staticmethod
     def backward(ctx, grad):
@@ -49,13 +55,13 @@ class _Traces(torch.autograd.Function):
             # the solver's checkpoints are those of a later forward: recompute ours
             _run_forward(op, ctx.coefs)
             op.last_forward = ctx.fid
-        g = op.solver.adjoint_run(grad.detach().to("cpu", torch.float64).numpy())
+        g = op.solver.adjoint_run_coefs(grad.detach().to("cpu", torch.float64).numpy())
         like = ctx.like
-        return tuple(torch.from_numpy(x).to(device=like.device, dtype=like.dtype) for x in g) + (None,)
+        return (None,) + tuple(torch.from_numpy(g[k]).to(device=like.device, dtype=like.dtype) for k in op.coefs)
 
 
-class AcousticTraces:
-    """traces = AcousticTraces(solver, nsteps)(kappa, binv_vx, binv_vy)"""
+class _Op:
+    coefs = ()
 
     def __init__(self, solver, nsteps, interval=0):
         self.solver = solver
@@ -64,8 +70,13 @@ class AcousticTraces:
         self.state0 = solver.checkpoint()
         self.last_forward = 0
 
+
+class AcousticTraces(_Op):
+    """traces = AcousticTraces(solver, nsteps)(kappa, binv_vx, binv_vy)"""
+    coefs = ACOUSTIC_COEFS
+
     def __call__(self, kappa, binv_vx, binv_vy):
-        return _Traces.apply(kappa, binv_vx, binv_vy, self)
+        return _Traces.apply(self, kappa, binv_vx, binv_vy)
 
 
 class AcousticTracesKappaRho(AcousticTraces):
@@ -74,4 +85,23 @@ class AcousticTracesKappaRho(AcousticTraces):
     the chain rule d/d rho = -(1/rho^2) d/d(1/rho) is applied by torch."""
 
     def __call__(self, kappa, rho_vx, rho_vy):
-        return _Traces.apply(kappa, 1.0 / rho_vx, 1.0 / rho_vy, self)
+        return _Traces.apply(self, kappa, 1.0 / rho_vx, 1.0 / rho_vy)
+
+
+class ElasticTraces(_Op):
+    """traces = ElasticTraces(solver, nsteps)(lam, twomu, mu, binv_vx, binv_vy):
+    lambda and 2 mu on MM, mu on NN, 1/rho on NM and MN, as the solver's
+    coefficient arrays."""
+    coefs = ELASTIC_COEFS
+
+    def __call__(self, lam, twomu, mu, binv_vx, binv_vy):
+        return _Traces.apply(self, lam, twomu, mu, binv_vx, binv_vy)
+
+
+class ElasticTracesLameRho(ElasticTraces):
+    """traces = ElasticTracesLameRho(solver, nsteps)(lam, mu_mm, mu_nn, rho_vx,
+    rho_vy): density and the Lame moduli, mu on the cells (MM, where the
+    solver keeps 2 mu) and on the nodes (NN); torch applies the chain rule."""
+
+    def __call__(self, lam, mu_mm, mu_nn, rho_vx, rho_vy):
+        return _Traces.apply(self, lam, 2.0 * mu_mm, mu_nn, 1.0 / rho_vx, 1.0 / rho_vy)
