@@ -1092,6 +1092,17 @@ __device__ __forceinline__ V4<ST> ld_v4s(PT p) {
   }
 }
 
+// one element through a global pointer
+template <class ST, class PT>
+__device__ __forceinline__ ST ld_1(PT p) {
+  typedef typename RawOf<sizeof(ST)>::e E;
+  typedef __attribute__((address_space(1))) const E* Q;
+  const E e = *(Q)p;
+  ST v;
+  __builtin_memcpy(&v, &e, sizeof v);
+  return v;
+}
+
 template <class ST, class PT>
 __device__ __forceinline__ void st_1(PT p, const ST& v) {
   typedef typename RawOf<sizeof(ST)>::e E;
@@ -1600,11 +1611,12 @@ __device__ __forceinline__ void store_f(const FR<ST>& X, int r, int c, const V4<
 // VGPRs) for elastic 16-bit storage and plain FP32, where latency hiding
 // pays; FP32 with a residual array three; FP64 keeps its registers.  The
 // acoustic 16-bit kernels issue half the loads per row, so they need more
-// waves in flight: six (at most 80 VGPRs; measured on the batched shots 12%
-// faster than four or five, eight spills)
+// waves in flight: eight (at most 64 VGPRs; on the 64 batched shots of
+// 512^2-2048^2, 0.726 ms per batched step against 0.747 at six and 0.768 at
+// five)
 template <class ST, int CP, int PH>
 constexpr int WPE = sizeof(ST) == 8 ? 1
-                    : sizeof(ST) == 2 && PH == TW_ACOUSTIC ? 6
+                    : sizeof(ST) == 2 && PH == TW_ACOUSTIC ? 8
                                                             : (sizeof(ST) == 4 && CP != TW_CPST_0 ? 3 : 4);
 
 constexpr int NXCD = 8;
@@ -1667,13 +1679,6 @@ __device__ __forceinline__ void seg_put4(CT* p, const V4<ST>& v) {
 #pragma unroll
   for (int j = 0; j < 4; j++) p[j] = (CT)St<ST>::ld(v.v[j]);
 }
-template <class ST, class CT, bool RF>
-__device__ __forceinline__ void seg_put4r(CT* p, const V4<ST>& v, uint32_t w) {
-  CT o[4];
-  cvt_rf<ST, CT, RF>(v, w, o);
-#pragma unroll
-  for (int j = 0; j < 4; j++) p[j] = o[j];
-}
 
 // core-column mask of a lane's four columns; true when all four are in
 __device__ __forceinline__ bool col_mask(int c, int clo, int chi, bool (&in)[4]) {
@@ -1688,6 +1693,46 @@ __device__ __forceinline__ bool col_mask(int c, int clo, int chi, bool (&in)[4])
 __device__ __forceinline__ int halo_col(int lane, int base, int cl, int cmax) {
   const int h = lane < 2 ? base - HALO + lane * 4 : (lane >= 62 ? base + STRIP + (lane - 62) * 4 : cl);
   return h < 0 ? 0 : (h > cmax ? cmax : h);
+}
+
+// Halo of the two LDS segments of a full-strip row, one value per lane:
+// lanes 0-15 hold segment A's 16 halo values (8 columns left of the strip, 8
+// right), lanes 16-31 segment B's (bodies with segment B only: lanes 0-15);
+// the other lanes load a value of the row and store nothing.  Columns are
+// clamped inside the row pitch (clamped values feed masked columns only).
+struct Halo1 {
+  int col;  // load column
+  int idx;  // segment index
+  int seg;  // 0: A, 1: B, -1: none
+};
+__device__ __forceinline__ Halo1 halo1(int lane, int base, int cmax, bool two) {
+  const int h = lane & 15;
+  const int c = h < 8 ? base - HALO + h : base + STRIP + h - 8;
+  Halo1 o;
+  o.col = c < 0 ? 0 : (c > cmax + 3 ? cmax + 3 : c);
+  o.idx = h < 8 ? h : HALO + STRIP + h - 8;
+  o.seg = two ? (lane < 16 ? 0 : (lane < 32 ? 1 : -1)) : (lane < 16 ? 1 : -1);
+  return o;
+}
+// this lane's halo value: row ra of A (segment A) or rb of B, plus its
+// decoded residual where the stencils read it (RF)
+template <class ST, class CT, bool RF>
+__device__ __forceinline__ CT halo_val(const FR<ST>& A, int ra, const FR<ST>& B, int rb, const Halo1& H) {
+  const bool a = H.seg == 0;
+  const ST v = ld_1<ST>((a ? A.row(ra) : B.row(rb)) + H.col);
+  CT o = (CT)St<ST>::ld(v);
+  if constexpr (RF && sizeof(ST) == 2) {
+    const int8_t q = ld_1<int8_t>((a ? A.brow(ra) : B.brow(rb)) + H.col);
+    o = o + (CT)k8_dec(__builtin_bit_cast(ST, (int16_t)q), v);
+  }
+  return o;
+}
+template <class CT>
+__device__ __forceinline__ void halo_put(CT* segA, CT* segB, const Halo1& H, CT v) {
+  if (H.seg == 0)
+    segA[H.idx] = v;
+  else if (H.seg == 1)
+    segB[H.idx] = v;
 }
 
 // store of a lane's four values, element-wise where only some columns are in
@@ -1915,6 +1960,10 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       ldx_f<true, ST, CT, RF>(SXY, x0, cl, perx, mx, B1);
       ldx_f<true, ST, CT, RF>(SXY, x0 + 1, cl, perx, mx, B2);
     }
+    // segment halos: 16-bit storage one value per lane (Halo1), wider types
+    // four values per edge lane (lanes 0-1 and 62-63)
+    constexpr bool H1 = sizeof(ST) == 2;
+    const Halo1 H = halo1(lane, base, cmax, PH == TW_ELASTIC);
     const int hc = halo_col(lane, base, cl, cmax);
     const bool hl = lane < 2 || lane >= 62;
     const int hs = lane < 2 ? lane * 4 : HALO + STRIP + (lane - 62) * 4;
@@ -1925,22 +1974,27 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
       const int rx = vx_row ? r : VX.rows - 1, ry = vy_row ? r : VY.rows - 1;
       // ---- loads
       const V4<ST> nA = ld_v4<ST>(PX.row(wrap_row(r + 1, perx, mx)) + cl);
-      V4<ST> nB, hA, oB;
+      V4<ST> nB, oB;
       if constexpr (PH == TW_ELASTIC) {
         nB = ld_v4<ST>(SXY.row(wrap_row(r + 2, perx, mx)) + cl);
-        hA = ld_v4<ST>(SXY.row(r) + hc);  // sxy row r halo
         oB = ld_v4<ST>(PY.row(ry) + cl);   // syy row r
       }
-      const V4<ST> hB = ld_v4<ST>(PY.row(ry) + hc);  // syy / p row r halo
-      uint32_t cnA = 0, cnB = 0, chA = 0, coB = 0, chB = 0;  // their residual codes (K8S)
+      // sxy row r / syy or p row r halo (acoustic: segment B only)
+      CT hv = (CT)0;
+      V4<ST> hA, hB;
+      if constexpr (H1) {
+        hv = halo_val<ST, CT, RF>(PH == TW_ELASTIC ? SXY : PY, r, PY, ry, H);
+      } else {
+        if constexpr (PH == TW_ELASTIC) hA = ld_v4<ST>(SXY.row(r) + hc);
+        hB = ld_v4<ST>(PY.row(ry) + hc);
+      }
+      uint32_t cnA = 0, cnB = 0, coB = 0;  // their residual codes (K8S)
       if constexpr (RF) {
         cnA = ld_c4<ST>(PX, wrap_row(r + 1, perx, mx), cl);
         if constexpr (PH == TW_ELASTIC) {
           cnB = ld_c4<ST>(SXY, wrap_row(r + 2, perx, mx), cl);
-          chA = ld_c4<ST>(SXY, r, hc);
           coB = ld_c4<ST>(PY, ry, cl);
         }
-        chB = ld_c4<ST>(PY, ry, hc);
       }
       const Q4<KT> kx = kq4<KT>(KBX, rx, cl);
       V4<ST> sx = ld_v4s<ST>(VX.row(rx) + cl);
@@ -1961,14 +2015,18 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
         } else {
           seg_own<CT>(segA[buf], b1, lane);
         }
-        if (hl) seg_put4r<ST, CT, RF>(segA[buf] + hs, hA, chA);
         CT o[4];
         cvt_rf<ST, CT, RF>(oB, coB, o);
         seg_own<CT>(segB[buf], o, lane);
       } else {
         seg_own<CT>(segB[buf], a2, lane);  // p row r
       }
-      if (hl) seg_put4r<ST, CT, RF>(segB[buf] + hs, hB, chB);
+      if constexpr (H1) {
+        halo_put<CT>(segA[buf], segB[buf], H, hv);
+      } else if (hl) {
+        if constexpr (PH == TW_ELASTIC) seg_put4<ST, CT>(segA[buf] + hs, hA);
+        seg_put4<ST, CT>(segB[buf] + hs, hB);
+      }
       __syncthreads();
       if (vx_row) {  // ---- vx (NM)
         CT b[4];
@@ -2263,6 +2321,9 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
     ldx_f<true, ST, CT, RF>(VX, x0 - 1, cl, perx, mx, B0);
     ldx_f<true, ST, CT, RF>(VX, x0, cl, perx, mx, B1);
     ldx_f<true, ST, CT, RF>(VX, x0 + 1, cl, perx, mx, B2);
+    // segment halos as in the velocity body
+    constexpr bool H1 = sizeof(ST) == 2;
+    const Halo1 H = halo1(lane, base, cmax, PH == TW_ELASTIC);
     const int hc = halo_col(lane, base, cl, cmax);
     const bool hl = lane < 2 || lane >= 62;
     const int hs = lane < 2 ? lane * 4 : HALO + STRIP + (lane - 62) * 4;
@@ -2272,11 +2333,10 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
       const bool mm_row = r < X0.rows;
       const int rm = mm_row ? r : X0.rows - 1;
       // ---- loads
-      V4<ST> nA, hA, sn, qn, oB;
+      V4<ST> nA, sn, qn, oB;
       Q4<KT> kn;
       if constexpr (PH == TW_ELASTIC) {
         nA = ld_v4<ST>(VY.row(wrap_row(r + 1, perx, mx)) + cl);
-        hA = ld_v4<ST>(VX.row(r) + hc);  // vx row r halo
         kn = kq4<KT>(KMU, r, cl);
         sn = ld_v4s<ST>(XN.row(r) + cl);
         qn = sn;
@@ -2285,17 +2345,21 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         oB = ld_v4<ST>(VY.row(rm) + cl);  // vy row r (no window in the acoustic case)
       }
       const V4<ST> nB = ld_v4<ST>(VX.row(wrap_row(r + 2, perx, mx)) + cl);
-      const V4<ST> hB = ld_v4<ST>(VY.row(rm) + hc);  // vy row r halo
-      uint32_t cnA = 0, cnB = 0, chA = 0, coB = 0, chB = 0;  // their residual codes (K8S)
+      CT hv = (CT)0;  // vx row r / vy row r halo
+      V4<ST> hA, hB;
+      if constexpr (H1) {
+        hv = halo_val<ST, CT, RF>(VX, r, VY, rm, H);
+      } else {
+        if constexpr (PH == TW_ELASTIC) hA = ld_v4<ST>(VX.row(r) + hc);
+        hB = ld_v4<ST>(VY.row(rm) + hc);
+      }
+      uint32_t cnA = 0, cnB = 0, coB = 0;  // their residual codes (K8S)
       if constexpr (RF) {
-        if constexpr (PH == TW_ELASTIC) {
+        if constexpr (PH == TW_ELASTIC)
           cnA = ld_c4<ST>(VY, wrap_row(r + 1, perx, mx), cl);
-          chA = ld_c4<ST>(VX, r, hc);
-        } else {
+        else
           coB = ld_c4<ST>(VY, rm, cl);
-        }
         cnB = ld_c4<ST>(VX, wrap_row(r + 2, perx, mx), cl);
-        chB = ld_c4<ST>(VY, rm, hc);
       }
       const Q4<KT> k0 = kq4<KT>(K0, rm, cl);
       Q4<KT> k1;
@@ -2319,14 +2383,18 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
         } else {
           seg_own<CT>(segA[buf], b1, lane);
         }
-        if (hl) seg_put4r<ST, CT, RF>(segA[buf] + hs, hA, chA);
         seg_own<CT>(segB[buf], a2, lane);  // vy row r
       } else {
         CT o[4];
         cvt_rf<ST, CT, RF>(oB, coB, o);
         seg_own<CT>(segB[buf], o, lane);
       }
-      if (hl) seg_put4r<ST, CT, RF>(segB[buf] + hs, hB, chB);
+      if constexpr (H1) {
+        halo_put<CT>(segA[buf], segB[buf], H, hv);
+      } else if (hl) {
+        if constexpr (PH == TW_ELASTIC) seg_put4<ST, CT>(segA[buf] + hs, hA);
+        seg_put4<ST, CT>(segB[buf] + hs, hB);
+      }
       __syncthreads();
       if constexpr (PH == TW_ELASTIC) {  // ---- sxy (NN); every core row is an NN row
         CT mu[4];
