@@ -204,3 +204,61 @@ def test_elastic_autograd_wrappers():
     fd = (loss(1) - loss(-1)) / (2 * eps)
     ad = sum(float((t.grad * torch.tensor(dd)).sum()) for t, dd in zip(args, d))
     assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+
+
+def test_elastic_adjoint_benchmark_model_512():
+    """The benchmark's elastic workload at 512^2 (layered model, 15 Hz Ricker
+    vertical force at 100 m depth, free-surface top, characteristic sides,
+    256 surface receivers on vx), 600 steps: the misfit
+    sum (trace - data)^2 against data from a model with 2% faster moduli in a
+    disc, differentiated by the adjoint (FP64) and by extrapolated central
+    differences along smooth relative perturbations of all five coefficient
+    arrays; FP32 storage's gradient against FP64's."""
+    import twave.workloads as W
+    m, nt = 512, 600
+    grads = {}
+    for prec in (T.FP64, T.FP32):
+        s = T.Solver(W.elastic_config(m=m, precision=prec, nt=nt + 8, dx_km=0.005))
+        W.setup_elastic(s, m, 0.005)
+        blob = s.checkpoint()
+        m0 = [s.get_coef(k) for k in COEFS]
+        x = np.arange(s.get_coef(T.C_LAMBDA).shape[0])[:, None]
+        y = np.arange(s.get_coef(T.C_LAMBDA).shape[1])[None, :]
+        disc = ((x - m // 4) ** 2 + (y - m // 2) ** 2 < (m // 10) ** 2) * 0.02 + 1.0
+        true = [mm * (disc if k == T.C_LAMBDA or k == T.C_TWOMU else 1.0) for k, mm in zip(COEFS, m0)]
+        data = traces(s, blob, true, nt)
+        s.restore(blob)
+        for k, v in zip(COEFS, m0):
+            s.set_coef(k, v)
+        s.adjoint_forward(nt)
+        tr = s.receivers()[:, -nt:]
+        g = s.adjoint_run_coefs(2 * (tr - data))
+        grads[prec] = g
+        print("precision", prec, "misfit %.6e" % np.sum((tr - data) ** 2), flush=True)
+        if prec == T.FP64:
+            rng = np.random.default_rng(21)
+            worst = 0.0
+            for _ in range(3):
+                d = []
+                for mm in m0:
+                    r, c = mm.shape
+                    kx, ky = rng.uniform(0.5, 3.0, 2)
+                    ph = rng.uniform(0, 2 * np.pi)
+                    d.append(mm * np.sin(2 * np.pi * (kx * np.arange(r)[:, None] / r + ky * np.arange(c)[None, :] / c) + ph))
+
+                def J(eps):
+                    return np.sum((traces(s, blob, [mm + eps * dd for mm, dd in zip(m0, d)], nt) - data) ** 2)
+
+                e = 1e-4
+                c1 = (J(e) - J(-e)) / (2 * e)
+                c2 = (J(e / 2) - J(-e / 2)) / e
+                fd = (4 * c2 - c1) / 3
+                ad = sum(np.sum(g[k] * dd) for k, dd in zip(COEFS, d))
+                worst = max(worst, abs(fd - ad) / abs(fd))
+            print("worst rel err adjoint vs extrapolated differences %.3e" % worst, flush=True)
+            assert worst < 1e-6, worst
+        s.close()
+    for k in COEFS:
+        err = C.rel_l2(grads[T.FP32][k], grads[T.FP64][k])
+        print("fp32 vs fp64 gradient rel-L2", k, "%.3e" % err, flush=True)
+        assert err < 1e-3, (k, err)
