@@ -209,13 +209,13 @@ def test_elastic_autograd_wrappers():
 def test_elastic_adjoint_benchmark_model_512():
     """The benchmark's elastic workload at 512^2 (layered model, 15 Hz Ricker
     vertical force at 100 m depth, free-surface top, characteristic sides,
-    256 surface receivers on vx), 600 steps: the misfit
-    sum (trace - data)^2 against data from a model with 2% faster moduli in a
-    disc, differentiated by the adjoint (FP64) and by extrapolated central
+    256 surface receivers on vx), 1000 steps: the misfit
+    sum (trace - data)^2 against data from a model with 2% stiffer moduli in a
+    disc around the source, differentiated by the adjoint (FP64) and by extrapolated central
     differences along smooth relative perturbations of all five coefficient
     arrays; FP32 storage's gradient against FP64's."""
     import twave.workloads as W
-    m, nt = 512, 600
+    m, nt = 512, 1000
     grads = {}
     for prec in (T.FP64, T.FP32):
         s = T.Solver(W.elastic_config(m=m, precision=prec, nt=nt + 8, dx_km=0.005))
@@ -224,7 +224,7 @@ def test_elastic_adjoint_benchmark_model_512():
         m0 = [s.get_coef(k) for k in COEFS]
         x = np.arange(s.get_coef(T.C_LAMBDA).shape[0])[:, None]
         y = np.arange(s.get_coef(T.C_LAMBDA).shape[1])[None, :]
-        disc = ((x - m // 4) ** 2 + (y - m // 2) ** 2 < (m // 10) ** 2) * 0.02 + 1.0
+        disc = ((x - 60) ** 2 + (y - m // 2) ** 2 < 30 ** 2) * 0.02 + 1.0  # 300 m deep, radius 150 m
         true = [mm * (disc if k == T.C_LAMBDA or k == T.C_TWOMU else 1.0) for k, mm in zip(COEFS, m0)]
         data = traces(s, blob, true, nt)
         s.restore(blob)
@@ -234,7 +234,9 @@ def test_elastic_adjoint_benchmark_model_512():
         tr = s.receivers()[:, -nt:]
         g = s.adjoint_run_coefs(2 * (tr - data))
         grads[prec] = g
-        print("precision", prec, "misfit %.6e" % np.sum((tr - data) ** 2), flush=True)
+        misfit = np.sum((tr - data) ** 2)
+        print("precision", prec, "misfit %.6e, data energy %.6e" % (misfit, np.sum(data ** 2)), flush=True)
+        assert misfit > 1e-6 * np.sum(data ** 2)  # the perturbation reaches the receivers
         if prec == T.FP64:
             rng = np.random.default_rng(21)
             worst = 0.0
