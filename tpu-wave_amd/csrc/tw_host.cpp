@@ -619,6 +619,24 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   p.dx = c->dx;
   if (const char* kv = getenv("TW_KERNEL")) p.kver = atoi(kv) == 1 ? 1 : (atoi(kv) == 3 ? 3 : 2);
   p.txc = tw::default_chunk_rows(c->precision, c->cpst);
+  {
+    // a wave walks the rows of its chunk one after another, so a grid with few
+    // strips and chunks leaves the chip idle: shorter chunks (down to 4 rows)
+    // until the core launch has a wave for every wave slot the core kernels
+    // can hold (CUs x 4 SIMDs x their waves per SIMD: FP64 1, FP32 with a
+    // residual array 3, acoustic 16-bit 8, otherwise 4)
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
+      cus = 256;
+    const bool sixteen = c->precision == TW_BF16 || c->precision == TW_FP16;
+    const int wpe = c->precision == TW_FP64 ? 1
+                    : (sixteen && c->physics == TW_ACOUSTIC) ? 8
+                    : (c->precision == TW_FP32 && c->cpst != TW_CPST_0) ? 3 : 4;
+    const int64_t slots = (int64_t)cus * 4 * wpe;
+    const int64_t strips = ((int64_t)c->my + 1 + 255) / 256;
+    const int64_t want = ((int64_t)c->mx * strips + slots - 1) / slots;
+    if (want < p.txc) p.txc = (int32_t)std::max<int64_t>(4, want);
+  }
   if (const char* tx = getenv("TW_TXC"))
     if (atoi(tx) > 0) p.txc = atoi(tx);
   if (const char* mb = getenv("TW_MERGE_BANDS")) p.merge_bands = atoi(mb) != 0;
