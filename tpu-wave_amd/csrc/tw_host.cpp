@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <utility>
 #include <array>
+#include <map>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -1636,6 +1637,30 @@ int adjoint_sweep(tw_solver* s, const double* rcv_adjoint, double* const* out) {
     HIPCHK(s, hipMemcpyAsync(s->adj_inj, rcv_adjoint, (size_t)s->nrcv * n * 8, hipMemcpyHostToDevice, s->stream));
   s->adj.inj = s->adj_inj;
   s->adj.nt = n;
+  // receivers that share a grid point: chains in list order for the injection
+  const std::vector<tw::Receiver>* lists[2] = {&s->rcv_v, &s->rcv_s};
+  for (int g = 0; g < 2; g++) {
+    const std::vector<tw::Receiver>& L = *lists[g];
+    const size_t m = L.size();
+    std::vector<int32_t> lead(m + 1, 0), next(m + 1, -1);
+    std::map<std::array<int32_t, 3>, int> last;  // point -> last receiver seen there
+    for (size_t r = 0; r < m; r++) {
+      const std::array<int32_t, 3> key{L[r].field, L[r].ix, L[r].iy};
+      auto it = last.find(key);
+      if (it == last.end()) {
+        lead[r] = 1;
+      } else {
+        next[it->second] = (int32_t)r;
+      }
+      last[key] = (int)r;
+    }
+    dfree(s, (void**)&s->adj.lead[g]);
+    dfree(s, (void**)&s->adj.next[g]);
+    if ((e = dalloc(s, (void**)&s->adj.lead[g], (m + 1) * 4))) return e;
+    if ((e = dalloc(s, (void**)&s->adj.next[g], (m + 1) * 4))) return e;
+    HIPCHK(s, hipMemcpyAsync(s->adj.lead[g], lead.data(), (m + 1) * 4, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->adj.next[g], next.data(), (m + 1) * 4, hipMemcpyHostToDevice, s->stream));
+  }
   const size_t as = arith_size(s->cfg.precision);
   for (int f : adj_fields(s))
     HIPCHK(s, hipMemsetAsync(s->adj.mu[f], 0, (size_t)p.f[f].rows * p.f[f].pitch * as, s->stream));

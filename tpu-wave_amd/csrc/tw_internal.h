@@ -162,6 +162,10 @@ struct AdjPlan {
   TapList dt[2];            // transpose of the M-output derivative of N data (x, y)
   const double* inj = nullptr;  // receiver adjoint sources [nrcv][nt]
   int32_t nt = 0;
+  // per receiver list (velocity, stress): 1 for the first receiver of its grid
+  // point, and the next receiver at the same point (-1: none)
+  int32_t* lead[2] = {};
+  int32_t* next[2] = {};
 };
 // forward fields of a saved state (the physics' storage arrays, field pitch)
 struct StateView {

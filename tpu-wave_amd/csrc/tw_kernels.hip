@@ -3382,6 +3382,8 @@ struct AdjArgs {
   int32_t nt, it_rel;
   const Receiver* rcv[2];
   int32_t nrcv[2];
+  const int32_t* lead[2];  // 1 for the first receiver of its grid point in list order
+  const int32_t* next[2];  // the next receiver at the same point, or -1
 };
 
 template <class ST, class KT, class CT>
@@ -3516,20 +3518,25 @@ __global__ void __launch_bounds__(256) k_adj3(Args<ST, KT, CT> U, AdjArgs<CT> J)
   }
 }
 
-// receiver sources of forward step it_rel (relative to the adjoint run); one
-// thread, since receivers may share a point
 struct Pitches {
   int64_t p[NF];
 };
+
+// receiver sources of forward step it_rel (relative to the adjoint run): one
+// thread per receiver; the first receiver of a grid point adds those of every
+// receiver at that point in list order (the chain), so the sums are those of
+// a sequential loop over the lists whatever the scheduling
 template <class CT>
-__global__ void k_adj_inject(AdjArgs<CT> J, Pitches P) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int g = 0; g < 2; g++)
-    for (int r = 0; r < J.nrcv[g]; r++) {
-      const Receiver rc = J.rcv[g][r];
-      CT& m = J.mu[rc.field][(int64_t)rc.ix * P.p[rc.field] + rc.iy];
-      m = m + (CT)J.inj[(int64_t)rc.row * J.nt + J.it_rel];
-    }
+__global__ void __launch_bounds__(256) k_adj_inject(AdjArgs<CT> J, Pitches P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = t < J.nrcv[0] ? 0 : 1;
+  const int r = g == 0 ? t : t - J.nrcv[0];
+  if (r >= J.nrcv[g] || !J.lead[g][r]) return;
+  const Receiver rc = J.rcv[g][r];
+  CT& m = J.mu[rc.field][(int64_t)rc.ix * P.p[rc.field] + rc.iy];
+  CT v = m;
+  for (int q = r; q >= 0; q = J.next[g][q]) v = v + (CT)J.inj[(int64_t)J.rcv[g][q].row * J.nt + J.it_rel];
+  m = v;
 }
 
 // ---- elastic
@@ -3813,6 +3820,10 @@ static hipError_t run_adjoint_step(const Plan& p, const AdjPlan& a, const StateV
   J.rcv[1] = p.rcv_s;
   J.nrcv[0] = p.nrcv_v;
   J.nrcv[1] = p.nrcv_s;
+  for (int g = 0; g < 2; g++) {
+    J.lead[g] = a.lead[g];
+    J.next[g] = a.next[g];
+  }
   StepCtl c;
   c.it = it;
   auto U1 = make_args<ST, KT, CT>(p, &c, nullptr);
@@ -3823,7 +3834,8 @@ static hipError_t run_adjoint_step(const Plan& p, const AdjPlan& a, const StateV
   }
   Pitches P;
   for (int f = 0; f < NF; f++) P.p[f] = p.f[f].pitch;
-  hipLaunchKernelGGL((k_adj_inject<CT>), dim3(1), dim3(64), 0, s, J, P);
+  const int nr = p.nrcv_v + p.nrcv_s;
+  if (nr > 0) hipLaunchKernelGGL((k_adj_inject<CT>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, J, P);
   const dim3 b(64, 4), g((unsigned)((p.my + 1 + 63) / 64), (unsigned)((p.mx + 1 + 3) / 4));
   const int n = (p.mx > p.my ? p.mx : p.my) + 1;
   const dim3 gl((unsigned)((n + 255) / 256));
