@@ -10,6 +10,14 @@ import twave as T
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True)
+def streaming_solo(monkeypatch):
+    """Solo runs on the streaming kernels the batch runs: on grids this small
+    tw_create would pick the per-point kernels, whose fields and traces are
+    the same bit for bit but whose energies sum their terms in another order."""
+    monkeypatch.setenv("TW_KERNEL", "2")
+
 # (mx, my, bc) of the problems: narrow and wide, SBP and periodic sides
 SHAPES = [(40, 48, "FCCC"), (64, 96, "CCCC"), (33, 70, "PPPP"), (80, 37, "FFPP"), (20, 300, "CCFF"),
           (48, 264, "PPCC")]

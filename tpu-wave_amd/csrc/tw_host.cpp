@@ -304,6 +304,9 @@ struct tw_solver {
   uint64_t gen = 0;
   bool ep_valid = false;
   bool need_uhat = false;
+  // small grid: single steps run the per-point kernels (tw_create's choice,
+  // not a TW_KERNEL request); batches still run the streaming kernels
+  bool auto_point = false;
   // discrete adjoint: checkpoints of the last checkpointed forward run
   tw::AdjPlan adj{};
   bool adj_taps = false;
@@ -583,6 +586,10 @@ double tw_ricker_increment(const tw_config* cfg, double f0, double t0, int32_t i
   return ricker_inc(cfg, f0, t0, it, half);
 }
 
+// grids of at most this many union points step with the per-point kernels
+// unless TW_KERNEL says otherwise (about 1200^2)
+constexpr int64_t kPointCells = 1500000;
+
 int tw_create(const tw_config* cfg, tw_solver** out) {
   if (!out) return fail(nullptr, TW_EINVAL, "null output handle");
   *out = nullptr;
@@ -618,7 +625,17 @@ int tw_create(const tw_config* cfg, tw_solver** out) {
   memcpy(p.bc, c->bc, 4);
   p.energy_form = c->energy_form;
   p.dx = c->dx;
-  if (const char* kv = getenv("TW_KERNEL")) p.kver = atoi(kv) == 1 ? 1 : (atoi(kv) == 3 ? 3 : 2);
+  if (const char* kv = getenv("TW_KERNEL")) {
+    p.kver = atoi(kv) == 1 ? 1 : (atoi(kv) == 3 ? 3 : 2);
+  } else if ((int64_t)(c->mx + 1) * (c->my + 1) <= kPointCells) {
+    // on grids this small a step is latency bound: the per-point kernels (one
+    // thread per point, all loads of a point in flight at once) beat waves
+    // walking rows (512^2 elastic BF16 + K8: 0.031 against 0.074 ms per
+    // step, 1024^2 0.068 against 0.080; at 2048^2 the streaming kernels run
+    // 0.106 against 0.198)
+    p.kver = 1;
+    s->auto_point = true;
+  }
   p.txc = tw::default_chunk_rows(c->precision, c->cpst);
   {
     // a wave walks the rows of its chunk one after another, so a grid with few
@@ -1817,11 +1834,13 @@ int batch_build(tw_batch* b, int nsteps) {
   b->halves.clear();
   // the batch's tile height for the geometry; each solver's own is restored
   // below, so its single-problem steps keep their partial-sum layout
-  std::vector<int> txc0(n), nparts(n), has_core(n);
+  std::vector<int> txc0(n), kver0(n), nparts(n), has_core(n);
   for (int k = 0; k < n; k++) {
     tw_solver* s = b->sv[k];
     txc0[k] = s->plan.txc;
     s->plan.txc = b->rows;
+    kver0[k] = s->plan.kver;
+    s->plan.kver = 2;  // the batch runs the streaming kernels for every problem
     nparts[k] = tw::half_parts(s->plan);
     tw::BatchCore c{};
     tw::BatchBand r{};
@@ -1830,7 +1849,10 @@ int batch_build(tw_batch* b, int nsteps) {
     has_core[k] = nc > 0;
   }
   auto restore_txc = [&]() {
-    for (int k = 0; k < n; k++) b->sv[k]->plan.txc = txc0[k];
+    for (int k = 0; k < n; k++) {
+      b->sv[k]->plan.txc = txc0[k];
+      b->sv[k]->plan.kver = kver0[k];
+    }
   };
   for (int k = 0; k < n; k++) {
     tw_solver* s = b->sv[k];
@@ -2051,7 +2073,7 @@ int tw_batch_create(tw_solver* const* solvers, int32_t n, tw_batch** out) {
       return fail(nullptr, TW_EINVAL, "batch: solvers must share physics, precision, coef16, cpst, device, energy form");
     if (s->need_uhat) return fail(nullptr, TW_EINVAL, "batch: characteristic sides of elastic problems are not batched");
     if (s->plan.fused) return fail(nullptr, TW_EINVAL, "batch: fused-step solvers are not batched");
-    if (s->plan.kver == 1 || c.cpst == TW_CPST_3C || c.cpst == TW_CPST_M3C || c.cpst == TW_CPST_6C)
+    if ((s->plan.kver == 1 && !s->auto_point) || c.cpst == TW_CPST_3C || c.cpst == TW_CPST_M3C || c.cpst == TW_CPST_6C)
       return fail(nullptr, TW_EINVAL, "batch: streaming kernels only (no 3C/-3C/6C)");
   }
   (void)hipSetDevice(c0.device);
