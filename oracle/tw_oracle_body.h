@@ -10,6 +10,19 @@
 #define OR_CAT(a, b) OR_CAT2(a, b)
 #define FN(name) OR_CAT(PFX, name)
 
+/* one multiply-accumulate a + x*c of the derivative sums and of their
+ * products with the material coefficient (dx_term, dy_term, mm_term).
+ * REFMODE (FP64, FP32): a separate product and sum, as the reference's host
+ * code computes it.  16-bit storage: one fused multiply-add (fmaf, correctly
+ * rounded), as nvcc contracts the reference's device loops
+ * (cuda/src/cuda/grid_kernel_x.cut:38-40, v_d += S * c). */
+#undef OR_MAC
+#if REFMODE
+#define OR_MAC(a, x, c) ((a) + (x) * (c))
+#else
+#define OR_MAC(a, x, c) fmaf((x), (c), (a))
+#endif
+
 typedef struct {
   int rows, cols;
   SREAL* s;   /* state                                   */
@@ -247,7 +260,7 @@ static void FN(acc_init)(FN(st_t) * S, int f) {
 /* x-derivative of partner field pf into acc of field f, times coefficient
  * (grid_drvt.tpp:368-464 for SBP rows, :45-167 structure for periodic; the
  * per-point order is v = 0; v = v + S*c per tap; R = R + v*P, drvt_to_rths
- * grid_drvt.tpp:23-34).  extra: MM elastic, the same v also feeds the second
+ * grid_drvt.tpp:23-34; each step one OR_MAC).  extra: MM elastic, the same v also feeds the second
  * stress with lambda and the first (dir 0) / second (dir 1) with 2 mu. */
 static void FN(dx_term)(FN(st_t) * S, int f, int pf, int coefk, int out_t) {
   FN(fld)* F = &S->f[f];
@@ -265,8 +278,8 @@ static void FN(dx_term)(FN(st_t) * S, int f, int pf, int coefk, int out_t) {
     const CREAL* pr = cp + (size_t)ix * cols;
     for (int iy = 0; iy < cols; iy++) {
       CREAL v = 0;
-      for (int k = 0; k < nt; k++) v = v + FN(pv)(P, (size_t)id[k] * P->cols + iy, rf) * cf[k];
-      a[iy] = a[iy] + v * pr[iy];
+      for (int k = 0; k < nt; k++) v = OR_MAC(v, FN(pv)(P, (size_t)id[k] * P->cols + iy, rf), cf[k]);
+      a[iy] = OR_MAC(a[iy], v, pr[iy]);
     }
   }
 }
@@ -288,8 +301,8 @@ static void FN(dy_term)(FN(st_t) * S, int f, int pf, int coefk, int out_t) {
       const int* id = T->idx + 5 * iy;
       const CREAL* cf = T->cf + 5 * iy;
       CREAL v = 0;
-      for (int k = 0; k < nt; k++) v = v + FN(pv)(P, p0 + id[k], rf) * cf[k];
-      a[iy] = a[iy] + v * pr[iy];
+      for (int k = 0; k < nt; k++) v = OR_MAC(v, FN(pv)(P, p0 + id[k], rf), cf[k]);
+      a[iy] = OR_MAC(a[iy], v, pr[iy]);
     }
   }
 }
@@ -313,14 +326,14 @@ static void FN(mm_term)(FN(st_t) * S, int pf, int dir) {
       const CREAL* cf = T->cf + 5 * i;
       CREAL v = 0;
       if (dir == 0)
-        for (int k = 0; k < nt; k++) v = v + FN(pv)(P, (size_t)id[k] * P->cols + iy, rf) * cf[k];
+        for (int k = 0; k < nt; k++) v = OR_MAC(v, FN(pv)(P, (size_t)id[k] * P->cols + iy, rf), cf[k]);
       else
-        for (int k = 0; k < nt; k++) v = v + FN(pv)(P, (size_t)ix * P->cols + id[k], rf) * cf[k];
+        for (int k = 0; k < nt; k++) v = OR_MAC(v, FN(pv)(P, (size_t)ix * P->cols + id[k], rf), cf[k]);
       size_t q = (size_t)ix * cols + iy;
-      A->acc[q] = A->acc[q] + v * lam[q];
-      B->acc[q] = B->acc[q] + v * lam[q];
-      if (dir == 0) A->acc[q] = A->acc[q] + v * tmu[q];
-      else          B->acc[q] = B->acc[q] + v * tmu[q];
+      A->acc[q] = OR_MAC(A->acc[q], v, lam[q]);
+      B->acc[q] = OR_MAC(B->acc[q], v, lam[q]);
+      if (dir == 0) A->acc[q] = OR_MAC(A->acc[q], v, tmu[q]);
+      else          B->acc[q] = OR_MAC(B->acc[q], v, tmu[q]);
     }
   }
 }

@@ -328,11 +328,26 @@ static Args<ST, KT, CT> make_args(const Plan& p, const StepCtl* c, double* part)
 }
 
 // ------------------------------------------------------------ derivatives
+// One multiply-accumulate step a + x * c of the derivative sums and of their
+// products with the material coefficient.  16-bit storage (FM) computes it as
+// one fused multiply-add, as nvcc contracts the reference's device loops
+// (cuda/src/cuda/grid_kernel_x.cut:38-40 v_d += S * c, drvt_to_rths);
+// FP32 and FP64 storage keep the separate product and sum of the reference's
+// host code, whose printed outputs they reproduce digit for digit.
+template <bool FM, class CT>
+__device__ __forceinline__ CT mac(CT a, CT x, CT c) {
+  if constexpr (FM)
+    return __builtin_fmaf(x, c, a);
+  else
+    return a + x * c;
+}
+
 // Derivative along x onto output row ix of an output with n rows, type out_t
 // (1 = N, 0 = M), from partner P.  v = 0; v = v + S*c per tap in tap order.
 template <class ST, class CT>
 __device__ __forceinline__ CT ddx(const F<ST>& P, int out_t, int ix, int iy, int n, int periodic, int M,
                                   const TabT<CT>& T, int rf = 0) {
+  constexpr bool FM = sizeof(ST) == 2;
   CT v = (CT)0;
   if (periodic) {
     int r = ix + (out_t ? -2 : -1);
@@ -340,23 +355,23 @@ __device__ __forceinline__ CT ddx(const F<ST>& P, int out_t, int ix, int iy, int
     for (int k = 0; k < 4; k++) {
       int rr = r + k;
       rr = rr < 0 ? rr + M : (rr >= M ? rr - M : rr);
-      v = v + pval<ST, CT>(P.s, P.r, P.at(rr, iy), rf) * T.st[k];
+      v = mac<FM>(v, pval<ST, CT>(P.s, P.r, P.at(rr, iy), rf), T.st[k]);
     }
     return v;
   }
   const int nb = out_t ? 4 : 3;
   if (ix < nb) {
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, P.at(w, iy), rf) * (out_t ? T.DN_L[ix][w] : T.DM_L[ix][w]);
+    for (int w = 0; w < 5; w++) v = mac<FM>(v, pval<ST, CT>(P.s, P.r, P.at(w, iy), rf), (out_t ? T.DN_L[ix][w] : T.DM_L[ix][w]));
   } else if (ix >= n - nb) {
     const int r = ix - (n - nb);
     const int b = P.rows - 5;
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, P.at(b + w, iy), rf) * (out_t ? T.DN_R[r][w] : T.DM_R[r][w]);
+    for (int w = 0; w < 5; w++) v = mac<FM>(v, pval<ST, CT>(P.s, P.r, P.at(b + w, iy), rf), (out_t ? T.DN_R[r][w] : T.DM_R[r][w]));
   } else {
     int r = ix + (out_t ? -2 : -1);
 #pragma unroll
-    for (int k = 0; k < 4; k++) v = v + pval<ST, CT>(P.s, P.r, P.at(r + k, iy), rf) * T.st[k];
+    for (int k = 0; k < 4; k++) v = mac<FM>(v, pval<ST, CT>(P.s, P.r, P.at(r + k, iy), rf), T.st[k]);
   }
   return v;
 }
@@ -364,6 +379,7 @@ __device__ __forceinline__ CT ddx(const F<ST>& P, int out_t, int ix, int iy, int
 template <class ST, class CT>
 __device__ __forceinline__ CT ddy(const F<ST>& P, int out_t, int ix, int iy, int n, int periodic, int M,
                                   const TabT<CT>& T, int rf = 0) {
+  constexpr bool FM = sizeof(ST) == 2;
   const int64_t row = (int64_t)ix * P.pitch;
   CT v = (CT)0;
   if (periodic) {
@@ -372,23 +388,23 @@ __device__ __forceinline__ CT ddy(const F<ST>& P, int out_t, int ix, int iy, int
     for (int k = 0; k < 4; k++) {
       int cc = c + k;
       cc = cc < 0 ? cc + M : (cc >= M ? cc - M : cc);
-      v = v + pval<ST, CT>(P.s, P.r, row + cc, rf) * T.st[k];
+      v = mac<FM>(v, pval<ST, CT>(P.s, P.r, row + cc, rf), T.st[k]);
     }
     return v;
   }
   const int nb = out_t ? 4 : 3;
   if (iy < nb) {
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, row + w, rf) * (out_t ? T.DN_L[iy][w] : T.DM_L[iy][w]);
+    for (int w = 0; w < 5; w++) v = mac<FM>(v, pval<ST, CT>(P.s, P.r, row + w, rf), (out_t ? T.DN_L[iy][w] : T.DM_L[iy][w]));
   } else if (iy >= n - nb) {
     const int r = iy - (n - nb);
     const int b = P.cols - 5;
 #pragma unroll
-    for (int w = 0; w < 5; w++) v = v + pval<ST, CT>(P.s, P.r, row + b + w, rf) * (out_t ? T.DN_R[r][w] : T.DM_R[r][w]);
+    for (int w = 0; w < 5; w++) v = mac<FM>(v, pval<ST, CT>(P.s, P.r, row + b + w, rf), (out_t ? T.DN_R[r][w] : T.DM_R[r][w]));
   } else {
     int c = iy + (out_t ? -2 : -1);
 #pragma unroll
-    for (int k = 0; k < 4; k++) v = v + pval<ST, CT>(P.s, P.r, row + c + k, rf) * T.st[k];
+    for (int k = 0; k < 4; k++) v = mac<FM>(v, pval<ST, CT>(P.s, P.r, row + c + k, rf), T.st[k]);
   }
   return v;
 }
@@ -666,6 +682,7 @@ __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
   const int el = A.physics == TW_ELASTIC;
   const int kmode = kmode_of(A.cpst);
   const int rf = A.cpst == TW_CPST_K8S;  // stencils read value + residual
+  constexpr bool FM = sizeof(ST) == 2;
   double e = 0.0;
   // ---- vx on NM: d/dx of sxx (elastic) or p, + d/dy of sxy (elastic)
   {
@@ -677,11 +694,11 @@ __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
       CT acc = acc_init<ST, CT>(X, q, A.cpst);
       const int pxf = el ? TW_SXX : TW_P;
       CT d = ddx<ST, CT>(A.f[pxf], 1, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
-      acc = acc + d * b;
+      acc = mac<FM>(acc, d, b);
       if (!A.perx) acc = sat_add(A, acc, TW_VX, pxf, 0, 1, ix, iy, own, b);
       if (el) {
         d = ddy<ST, CT>(A.f[TW_SXY], 0, ix, iy, X.cols, A.pery, A.my, A.t, rf);
-        acc = acc + d * b;
+        acc = mac<FM>(acc, d, b);
         if (!A.pery) acc = sat_add(A, acc, TW_VX, TW_SXY, 1, 0, ix, iy, own, b);
       }
       ST sn = X.s[q], rn = sn;
@@ -704,12 +721,12 @@ __device__ __forceinline__ double vel_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
       CT acc = acc_init<ST, CT>(X, q, A.cpst);
       if (el) {
         CT d = ddx<ST, CT>(A.f[TW_SXY], 0, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
-        acc = acc + d * b;
+        acc = mac<FM>(acc, d, b);
         if (!A.perx) acc = sat_add(A, acc, TW_VY, TW_SXY, 0, 0, ix, iy, own, b);
       }
       const int pyf = el ? TW_SYY : TW_P;
       CT d = ddy<ST, CT>(A.f[pyf], 1, ix, iy, X.cols, A.pery, A.my, A.t, rf);
-      acc = acc + d * b;
+      acc = mac<FM>(acc, d, b);
       if (!A.pery) acc = sat_add(A, acc, TW_VY, pyf, 1, 1, ix, iy, own, b);
       ST sn = X.s[q], rn = sn;
       if (kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R) rn = rd_res(X.r, q, kmode);
@@ -750,6 +767,7 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
   const int kmode = kmode_of(A.cpst);
   const int rf = A.cpst == TW_CPST_K8S;  // stencils read value + residual
   const int rmode = kmode || A.cpst == TW_CPST_3R || A.cpst == TW_CPST_M3R || A.cpst == TW_CPST_6R;
+  constexpr bool FM = sizeof(ST) == 2;
   double e = 0.0;
   if (A.physics == TW_ELASTIC) {
     {
@@ -759,9 +777,9 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         const CT mu = kld<KT, CT>(A.k[TW_C_MU].get(ix, iy));
         CT acc = acc_init<ST, CT>(X, q, A.cpst);
         CT d = ddx<ST, CT>(A.f[TW_VY], 1, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
-        acc = acc + d * mu;
+        acc = mac<FM>(acc, d, mu);
         d = ddy<ST, CT>(A.f[TW_VX], 1, ix, iy, X.cols, A.pery, A.my, A.t, rf);
-        acc = acc + d * mu;
+        acc = mac<FM>(acc, d, mu);
         ST sn = X.s[q], rn = sn;
         if (rmode) rn = rd_res(X.r, q, kmode);
         update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
@@ -782,13 +800,13 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
         CT a0 = acc_init<ST, CT>(X0, q, A.cpst);
         CT a1 = acc_init<ST, CT>(X1, q, A.cpst);
         CT v = ddx<ST, CT>(A.f[TW_VX], 0, ix, iy, X0.rows, A.perx, A.mx, A.t, rf);
-        a0 = a0 + v * lam;
-        a1 = a1 + v * lam;
-        a0 = a0 + v * tmu;
+        a0 = mac<FM>(a0, v, lam);
+        a1 = mac<FM>(a1, v, lam);
+        a0 = mac<FM>(a0, v, tmu);
         CT w = ddy<ST, CT>(A.f[TW_VY], 0, ix, iy, X0.cols, A.pery, A.my, A.t, rf);
-        a0 = a0 + w * lam;
-        a1 = a1 + w * lam;
-        a1 = a1 + w * tmu;
+        a0 = mac<FM>(a0, w, lam);
+        a1 = mac<FM>(a1, w, lam);
+        a1 = mac<FM>(a1, w, tmu);
         ST s0 = X0.s[q], r0 = s0, s1 = X1.s[q], r1 = s1;
         if (rmode) {
           r0 = rd_res(X0.r, q, kmode);
@@ -826,9 +844,9 @@ __device__ __forceinline__ double str_pt_e(const Stat<ST, KT, CT>& A, const Dyn&
       const CT kap = kld<KT, CT>(A.k[TW_C_KAPPA].get(ix, iy));
       CT acc = acc_init<ST, CT>(X, q, A.cpst);
       CT d = ddx<ST, CT>(A.f[TW_VX], 0, ix, iy, X.rows, A.perx, A.mx, A.t, rf);
-      acc = acc + d * kap;
+      acc = mac<FM>(acc, d, kap);
       d = ddy<ST, CT>(A.f[TW_VY], 0, ix, iy, X.cols, A.pery, A.my, A.t, rf);
-      acc = acc + d * kap;
+      acc = mac<FM>(acc, d, kap);
       ST sn = X.s[q], rn = sn;
       if (rmode) rn = rd_res(X.r, q, kmode);
       update<ST, CT>(X, q, acc, A.cpst, &sn, &rn);
@@ -1173,21 +1191,21 @@ __device__ __forceinline__ void seg_fill(CT* seg, const F<ST>& P, int row, int b
 }
 
 // interior x derivative from the register window (rows in tap order)
-template <class CT>
+template <bool FM, class CT>
 __device__ __forceinline__ CT dx_win(const CT w[4][4], int j, const CT st[4]) {
   CT v = (CT)0;
 #pragma unroll
-  for (int k = 0; k < 4; k++) v = v + w[k][j] * st[k];
+  for (int k = 0; k < 4; k++) v = mac<FM>(v, w[k][j], st[k]);
   return v;
 }
 
 // interior / periodic y derivative from the LDS segment
-template <class CT>
+template <bool FM, class CT>
 __device__ __forceinline__ CT dy_win(const CT* seg, int base, int col, int out_t, const CT st[4]) {
   CT v = (CT)0;
   const int p = col - base + HALO + (out_t ? -2 : -1);
 #pragma unroll
-  for (int k = 0; k < 4; k++) v = v + seg[p + k] * st[k];
+  for (int k = 0; k < 4; k++) v = mac<FM>(v, seg[p + k], st[k]);
   return v;
 }
 
@@ -1507,12 +1525,12 @@ __device__ __forceinline__ void seg12(const CT* seg, int lane, CT out[12]) {
 }
 
 // y derivative of column j (0..3) of the lane from seg12 values
-template <class CT>
+template <bool FM, class CT>
 __device__ __forceinline__ CT dy12(const CT v[12], int j, int out_t, const CT st[4]) {
   const int b = 4 + j + (out_t ? -2 : -1);
   CT d = (CT)0;
 #pragma unroll
-  for (int k = 0; k < 4; k++) d = d + v[b + k] * st[k];
+  for (int k = 0; k < 4; k++) d = mac<FM>(d, v[b + k], st[k]);
   return d;
 }
 
@@ -1749,14 +1767,14 @@ __device__ __forceinline__ void st_m(PT rowp, int c, const V4<ST>& v, bool full4
 
 // x derivative of column j from the window rows in tap order (same operation
 // order as dx_win)
-template <class CT>
+template <bool FM, class CT>
 __device__ __forceinline__ CT dx4(const CT (&w0)[4], const CT (&w1)[4], const CT (&w2)[4], const CT (&w3)[4], int j,
                                   const CT st[4]) {
   CT v = (CT)0;
-  v = v + w0[j] * st[0];
-  v = v + w1[j] * st[1];
-  v = v + w2[j] * st[2];
-  v = v + w3[j] * st[3];
+  v = mac<FM>(v, w0[j], st[0]);
+  v = mac<FM>(v, w1[j], st[1]);
+  v = mac<FM>(v, w2[j], st[2]);
+  v = mac<FM>(v, w3[j], st[3]);
   return v;
 }
 
@@ -1809,6 +1827,7 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
   constexpr bool RM = CPT<CP>::rm;
   constexpr int KM = CPT<CP>::km;
   constexpr bool RF = CPT<CP>::rf;
+  constexpr bool FM = sizeof(ST) == 2;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // sxy row (y partner of vx, elastic)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // syy or p row (y partner of vy)
   __shared__ double sh[8];
@@ -1894,8 +1913,8 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
-          acc = acc + dx_win<CT>(wa, j, st) * b[j];
-          if constexpr (PH == TW_ELASTIC) acc = acc + dy12<CT>(ya, j, 0, st) * b[j];
+          acc = mac<FM>(acc, dx_win<FM, CT>(wa, j, st), b[j]);
+          if constexpr (PH == TW_ELASTIC) acc = mac<FM>(acc, dy12<FM, CT>(ya, j, 0, st), b[j]);
           upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
         }
         src4<ST, CT, CP>(sn, rn, D, TW_GRID_NM, r, c);
@@ -1920,8 +1939,8 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
-          if constexpr (PH == TW_ELASTIC) acc = acc + dx_win<CT>(wb, j, st) * b[j];
-          acc = acc + dy12<CT>(yb, j, 1, st) * b[j];
+          if constexpr (PH == TW_ELASTIC) acc = mac<FM>(acc, dx_win<FM, CT>(wb, j, st), b[j]);
+          acc = mac<FM>(acc, dy12<FM, CT>(yb, j, 1, st), b[j]);
           upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
         }
         src4<ST, CT, CP>(sn, rn, D, TW_GRID_MN, r, c);
@@ -2039,8 +2058,8 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(qx.v[j], sx.v[j]);
-          acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * b[j];
-          if constexpr (PH == TW_ELASTIC) acc = acc + dy12<CT>(ya, j, 0, st) * b[j];
+          acc = mac<FM>(acc, dx4<FM, CT>(a0, a1, a2, a3, j, st), b[j]);
+          if constexpr (PH == TW_ELASTIC) acc = mac<FM>(acc, dy12<FM, CT>(ya, j, 0, st), b[j]);
           CT dq;
           upd_q<ST, CT, CP>(sx.v[j], qx.v[j], acc, dq);
           if (srow) src1<ST, CT, CP>(sx.v[j], qx.v[j], dq, D.src_inc, D.src_iy - c == j);
@@ -2061,8 +2080,8 @@ __device__ __forceinline__ void vel3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(qy.v[j], sy.v[j]);
-          if constexpr (PH == TW_ELASTIC) acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * b[j];
-          acc = acc + dy12<CT>(yb, j, 1, st) * b[j];
+          if constexpr (PH == TW_ELASTIC) acc = mac<FM>(acc, dx4<FM, CT>(b0, b1, b2, b3, j, st), b[j]);
+          acc = mac<FM>(acc, dy12<FM, CT>(yb, j, 1, st), b[j]);
           CT dq;
           upd_q<ST, CT, CP>(sy.v[j], qy.v[j], acc, dq);
           if (srow) src1<ST, CT, CP>(sy.v[j], qy.v[j], dq, D.src_inc, D.src_iy - c == j);
@@ -2129,6 +2148,7 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
   constexpr bool RM = CPT<CP>::rm;
   constexpr int KM = CPT<CP>::km;
   constexpr bool RF = CPT<CP>::rf;
+  constexpr bool FM = sizeof(ST) == 2;
   __shared__ __attribute__((aligned(16))) CT segA[2][SEG];  // vx row (y partner of sxy)
   __shared__ __attribute__((aligned(16))) CT segB[2][SEG];  // vy row (y partner of MM / p)
   __shared__ double sh[8];
@@ -2218,8 +2238,8 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
-            acc = acc + dx_win<CT>(wa, j, st) * mu[j];
-            acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
+            acc = mac<FM>(acc, dx_win<FM, CT>(wa, j, st), mu[j]);
+            acc = mac<FM>(acc, dy12<FM, CT>(ya, j, 1, st), mu[j]);
             upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
           }
           src4<ST, CT, CP>(sn, rn, D, TW_GRID_NN, r, c);
@@ -2252,14 +2272,14 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           for (int j = 0; j < 4; j++) {
             CT a0 = acc0<ST, CT, CP>(m0.v[j], n0.v[j]);
             CT a1 = acc0<ST, CT, CP>(m1.v[j], n1.v[j]);
-            const CT v = dx_win<CT>(wb, j, st);
-            a0 = a0 + v * lam[j];
-            a1 = a1 + v * lam[j];
-            a0 = a0 + v * tmu[j];
-            const CT w = dy12<CT>(yb, j, 0, st);
-            a0 = a0 + w * lam[j];
-            a1 = a1 + w * lam[j];
-            a1 = a1 + w * tmu[j];
+            const CT v = dx_win<FM, CT>(wb, j, st);
+            a0 = mac<FM>(a0, v, lam[j]);
+            a1 = mac<FM>(a1, v, lam[j]);
+            a0 = mac<FM>(a0, v, tmu[j]);
+            const CT w = dy12<FM, CT>(yb, j, 0, st);
+            a0 = mac<FM>(a0, w, lam[j]);
+            a1 = mac<FM>(a1, w, lam[j]);
+            a1 = mac<FM>(a1, w, tmu[j]);
             upd<ST, CT, CP>(n0.v[j], m0.v[j], a0);
             upd<ST, CT, CP>(n1.v[j], m1.v[j], a1);
           }
@@ -2284,8 +2304,8 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             CT acc = acc0<ST, CT, CP>(rn.v[j], sn.v[j]);
-            acc = acc + dx_win<CT>(wb, j, st) * kap[j];
-            acc = acc + dy12<CT>(yb, j, 0, st) * kap[j];
+            acc = mac<FM>(acc, dx_win<FM, CT>(wb, j, st), kap[j]);
+            acc = mac<FM>(acc, dy12<FM, CT>(yb, j, 0, st), kap[j]);
             upd<ST, CT, CP>(sn.v[j], rn.v[j], acc);
           }
           src4<ST, CT, CP>(sn, rn, D, TW_GRID_MM, r, c);
@@ -2407,8 +2427,8 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(qn.v[j], sn.v[j]);
-          acc = acc + dx4<CT>(a0, a1, a2, a3, j, st) * mu[j];
-          acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
+          acc = mac<FM>(acc, dx4<FM, CT>(a0, a1, a2, a3, j, st), mu[j]);
+          acc = mac<FM>(acc, dy12<FM, CT>(ya, j, 1, st), mu[j]);
           CT dq;
           upd_q<ST, CT, CP>(sn.v[j], qn.v[j], acc, dq);
           if (srow) src1<ST, CT, CP>(sn.v[j], qn.v[j], dq, D.src_inc, D.src_iy - c == j);
@@ -2432,14 +2452,14 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
           for (int j = 0; j < 4; j++) {
             CT a0 = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
             CT a1 = acc0<ST, CT, CP>(q1.v[j], s1.v[j]);
-            const CT v = dx4<CT>(b0, b1, b2, b3, j, st);
-            a0 = a0 + v * lam[j];
-            a1 = a1 + v * lam[j];
-            a0 = a0 + v * tmu[j];
-            const CT w = dy12<CT>(yb, j, 0, st);
-            a0 = a0 + w * lam[j];
-            a1 = a1 + w * lam[j];
-            a1 = a1 + w * tmu[j];
+            const CT v = dx4<FM, CT>(b0, b1, b2, b3, j, st);
+            a0 = mac<FM>(a0, v, lam[j]);
+            a1 = mac<FM>(a1, v, lam[j]);
+            a0 = mac<FM>(a0, v, tmu[j]);
+            const CT w = dy12<FM, CT>(yb, j, 0, st);
+            a0 = mac<FM>(a0, w, lam[j]);
+            a1 = mac<FM>(a1, w, lam[j]);
+            a1 = mac<FM>(a1, w, tmu[j]);
             CT d0, d1;
             upd_q<ST, CT, CP>(s0.v[j], q0.v[j], a0, d0);
             upd_q<ST, CT, CP>(s1.v[j], q1.v[j], a1, d1);
@@ -2467,8 +2487,8 @@ __device__ __forceinline__ void str3_body(const Stat<ST, KT, CT>* __restrict__ S
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             CT acc = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
-            acc = acc + dx4<CT>(b0, b1, b2, b3, j, st) * kap[j];
-            acc = acc + dy12<CT>(yb, j, 0, st) * kap[j];
+            acc = mac<FM>(acc, dx4<FM, CT>(b0, b1, b2, b3, j, st), kap[j]);
+            acc = mac<FM>(acc, dy12<FM, CT>(yb, j, 0, st), kap[j]);
             CT dq;
             upd_q<ST, CT, CP>(s0.v[j], q0.v[j], acc, dq);
             if (srow) src1<ST, CT, CP>(s0.v[j], q0.v[j], dq, D.src_inc, D.src_iy - c == j);
@@ -2809,6 +2829,7 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
                                            int by, int nbx) {
   constexpr bool RM = CPT<CP>::rm;
   constexpr int KM = CPT<CP>::km;
+  constexpr bool FM = sizeof(ST) == 2;
   using S = St<ST>;
   __shared__ __attribute__((aligned(16))) CT segA[SEG];  // old sxy row v (y partner of vx)
   __shared__ __attribute__((aligned(16))) CT segB[SEG];  // old syy row v (y partner of vy)
@@ -2944,8 +2965,8 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         CT acc = acc0<ST, CT, CP>(qx.v[j], sx.v[j]);
-        acc = acc + dx4<CT>(w0, w1, w2, w3, j, st) * bxv[j];
-        acc = acc + dy12<CT>(ya, j, 0, st) * bxv[j];
+        acc = mac<FM>(acc, dx4<FM, CT>(w0, w1, w2, w3, j, st), bxv[j]);
+        acc = mac<FM>(acc, dy12<FM, CT>(ya, j, 0, st), bxv[j]);
         upd<ST, CT, CP>(sx.v[j], qx.v[j], acc);
       }
       cvt_v4<ST, CT>(ringB[slot5(v - 1)][lane], w0);
@@ -2955,8 +2976,8 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         CT acy = acc0<ST, CT, CP>(qy.v[j], sy.v[j]);
-        acy = acy + dx4<CT>(w0, w1, w2, w3, j, st) * byv[j];
-        acy = acy + dy12<CT>(yb, j, 1, st) * byv[j];
+        acy = mac<FM>(acy, dx4<FM, CT>(w0, w1, w2, w3, j, st), byv[j]);
+        acy = mac<FM>(acy, dy12<FM, CT>(yb, j, 1, st), byv[j]);
         upd<ST, CT, CP>(sy.v[j], qy.v[j], acy);
       }
       src4<ST, CT, CP>(sx, qx, D, TW_GRID_NM, v, c);
@@ -3011,8 +3032,8 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           CT acc = acc0<ST, CT, CP>(qn.v[j], sn.v[j]);
-          acc = acc + dx4<CT>(w0, w1, w2, w3, j, st) * mu[j];
-          acc = acc + dy12<CT>(ya, j, 1, st) * mu[j];
+          acc = mac<FM>(acc, dx4<FM, CT>(w0, w1, w2, w3, j, st), mu[j]);
+          acc = mac<FM>(acc, dy12<FM, CT>(ya, j, 1, st), mu[j]);
           upd<ST, CT, CP>(sn.v[j], qn.v[j], acc);
         }
         src4<ST, CT, CP>(sn, qn, D, TW_GRID_NN, s, c);
@@ -3037,14 +3058,14 @@ __device__ __forceinline__ void fused_body(const Stat<ST, KT, CT>* __restrict__ 
         for (int j = 0; j < 4; j++) {
           CT e0 = acc0<ST, CT, CP>(q0.v[j], s0.v[j]);
           CT e1 = acc0<ST, CT, CP>(q1.v[j], s1.v[j]);
-          const CT vv = dx4<CT>(w0, w1, w2, w3, j, st);
-          e0 = e0 + vv * lam[j];
-          e1 = e1 + vv * lam[j];
-          e0 = e0 + vv * tmu[j];
-          const CT w = dy12<CT>(yb, j, 0, st);
-          e0 = e0 + w * lam[j];
-          e1 = e1 + w * lam[j];
-          e1 = e1 + w * tmu[j];
+          const CT vv = dx4<FM, CT>(w0, w1, w2, w3, j, st);
+          e0 = mac<FM>(e0, vv, lam[j]);
+          e1 = mac<FM>(e1, vv, lam[j]);
+          e0 = mac<FM>(e0, vv, tmu[j]);
+          const CT w = dy12<FM, CT>(yb, j, 0, st);
+          e0 = mac<FM>(e0, w, lam[j]);
+          e1 = mac<FM>(e1, w, lam[j]);
+          e1 = mac<FM>(e1, w, tmu[j]);
           upd<ST, CT, CP>(s0.v[j], q0.v[j], e0);
           upd<ST, CT, CP>(s1.v[j], q1.v[j], e1);
         }
