@@ -71,7 +71,7 @@ struct bf16 {
   uint16_t u;
 };
 // 8-bit material coefficient: a code q in [0, 255] of its row's affine map
-// value = off + q * step (FP32, two roundings); c8f is a decoded value
+// value = off + q * step (FP32, one fused multiply-add); c8f is a decoded value
 struct c8 {
   uint8_t q;
 };
