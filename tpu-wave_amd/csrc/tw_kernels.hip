@@ -3412,6 +3412,18 @@ __device__ __forceinline__ CT kv(const Stat<ST, KT, CT>& A, int k, int ix, int i
   return kld<KT, CT>(A.k[k].get(ix, iy));
 }
 
+// A partner index along a non-periodic axis whose transposed tap list is the
+// interior one: indices [5, len - 6] of a partner array of len points.  Its
+// list (built on the host in ascending output order) holds the four outputs
+// first + k, k = 0..3, with coefficients st[3 - k]: first = index - 2 for the
+// M-output transpose (dt: an N partner read by M outputs i - 1 .. i + 2),
+// index - 1 for the N-output transpose (gt: an M partner read by N outputs
+// i - 2 .. i + 1).  The kernels then sum those four terms in the same order
+// without reading the list.
+__device__ __forceinline__ bool tap_interior(int index, int len, int periodic) {
+  return !periodic && index >= 5 && index <= len - 6;
+}
+
 // K1
 template <class ST, class KT, class CT>
 __global__ void __launch_bounds__(256) k_adj1(Args<ST, KT, CT> U, AdjArgs<CT> J) {
@@ -3427,20 +3439,36 @@ __global__ void __launch_bounds__(256) k_adj1(Args<ST, KT, CT> U, AdjArgs<CT> J)
   const F<ST>& X = U.f[TW_VX];
   if (ix < X.rows && iy < X.cols) {  // mu_vx += Dx^T (kappa mu_p), along x
     CT acc = (CT)0;
-    const TapT<CT>& L = J.dt[0];
-    for (int e = 0; e < L.n[ix]; e++) {
-      const int i = L.idx[ix * TMAX + e];
-      acc = acc + L.cf[ix * TMAX + e] * (kv(U, TW_C_KAPPA, i, iy) * J.mu[TW_P][(int64_t)i * pp + iy]);
+    if (tap_interior(ix, X.rows, U.perx)) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int i = ix - 2 + k;
+        acc = acc + U.t.st[3 - k] * (kv(U, TW_C_KAPPA, i, iy) * J.mu[TW_P][(int64_t)i * pp + iy]);
+      }
+    } else {
+      const TapT<CT>& L = J.dt[0];
+      for (int e = 0; e < L.n[ix]; e++) {
+        const int i = L.idx[ix * TMAX + e];
+        acc = acc + L.cf[ix * TMAX + e] * (kv(U, TW_C_KAPPA, i, iy) * J.mu[TW_P][(int64_t)i * pp + iy]);
+      }
     }
     J.mu[TW_VX][(int64_t)ix * X.pitch + iy] += acc;
   }
   const F<ST>& Y = U.f[TW_VY];
   if (ix < Y.rows && iy < Y.cols) {  // mu_vy += Dy^T (kappa mu_p), along y
     CT acc = (CT)0;
-    const TapT<CT>& L = J.dt[1];
-    for (int e = 0; e < L.n[iy]; e++) {
-      const int j = L.idx[iy * TMAX + e];
-      acc = acc + L.cf[iy * TMAX + e] * (kv(U, TW_C_KAPPA, ix, j) * J.mu[TW_P][(int64_t)ix * pp + j]);
+    if (tap_interior(iy, Y.cols, U.pery)) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int j = iy - 2 + k;
+        acc = acc + U.t.st[3 - k] * (kv(U, TW_C_KAPPA, ix, j) * J.mu[TW_P][(int64_t)ix * pp + j]);
+      }
+    } else {
+      const TapT<CT>& L = J.dt[1];
+      for (int e = 0; e < L.n[iy]; e++) {
+        const int j = L.idx[iy * TMAX + e];
+        acc = acc + L.cf[iy * TMAX + e] * (kv(U, TW_C_KAPPA, ix, j) * J.mu[TW_P][(int64_t)ix * pp + j]);
+      }
     }
     J.mu[TW_VY][(int64_t)ix * Y.pitch + iy] += acc;
   }
@@ -3479,15 +3507,31 @@ __global__ void __launch_bounds__(256) k_adj2(Args<ST, KT, CT> U, AdjArgs<CT> J)
   }
   if (ix < P.rows && iy < P.cols) {  // mu_p += G^T (b mu_v) + SAT^T
     CT acc = (CT)0;
-    const TapT<CT>& Lx = J.gt[0];
-    for (int e = 0; e < Lx.n[ix]; e++) {
-      const int j = Lx.idx[ix * TMAX + e];
-      acc = acc + Lx.cf[ix * TMAX + e] * (kv(U, TW_C_BINV_VX, j, iy) * J.mu[TW_VX][X.at(j, iy)]);
+    if (tap_interior(ix, P.rows, U.perx)) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int j = ix - 1 + k;
+        acc = acc + U.t.st[3 - k] * (kv(U, TW_C_BINV_VX, j, iy) * J.mu[TW_VX][X.at(j, iy)]);
+      }
+    } else {
+      const TapT<CT>& Lx = J.gt[0];
+      for (int e = 0; e < Lx.n[ix]; e++) {
+        const int j = Lx.idx[ix * TMAX + e];
+        acc = acc + Lx.cf[ix * TMAX + e] * (kv(U, TW_C_BINV_VX, j, iy) * J.mu[TW_VX][X.at(j, iy)]);
+      }
     }
-    const TapT<CT>& Ly = J.gt[1];
-    for (int e = 0; e < Ly.n[iy]; e++) {
-      const int j = Ly.idx[iy * TMAX + e];
-      acc = acc + Ly.cf[iy * TMAX + e] * (kv(U, TW_C_BINV_VY, ix, j) * J.mu[TW_VY][Y.at(ix, j)]);
+    if (tap_interior(iy, P.cols, U.pery)) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int j = iy - 1 + k;
+        acc = acc + U.t.st[3 - k] * (kv(U, TW_C_BINV_VY, ix, j) * J.mu[TW_VY][Y.at(ix, j)]);
+      }
+    } else {
+      const TapT<CT>& Ly = J.gt[1];
+      for (int e = 0; e < Ly.n[iy]; e++) {
+        const int j = Ly.idx[iy * TMAX + e];
+        acc = acc + Ly.cf[iy * TMAX + e] * (kv(U, TW_C_BINV_VY, ix, j) * J.mu[TW_VY][Y.at(ix, j)]);
+      }
     }
     // SAT: the boundary velocity point reads the 3-point extrapolation of p
     if (!U.perx) {
