@@ -303,6 +303,9 @@ struct tw_solver {
   // receivers, coefficients): a batch rebuilds its graph when it differs
   uint64_t gen = 0;
   bool ep_valid = false;
+  // steps recomputed inside the adjoint sweep: their energies and receiver
+  // values were recorded by the forward run, so these steps record nothing
+  bool quiet = false;
   bool need_uhat = false;
   // small grid: single steps run the per-point kernels (tw_create's choice,
   // not a TW_KERNEL request); batches still run the streaming kernels
@@ -510,7 +513,7 @@ tw::Pending stress_pending(tw_solver* s, int it_done, bool energy) {
     }
     p.scale = c->energy_form == 1 ? 1.0 : 0.5;
   }
-  if (it_done < c->nt_max && !s->rcv_s.empty()) {
+  if (!s->quiet && it_done < c->nt_max && !s->rcv_s.empty()) {
     p.rcv = s->plan.rcv_s;
     p.nrcv = (int32_t)s->rcv_s.size();
     p.rcv_it = it_done;
@@ -952,7 +955,7 @@ int tw_step(tw_solver* s, int32_t nsteps) {
   if (p.fused) {
     for (int k = 0; k < nsteps; k++) {
       const int it = s->it;
-      const bool rec = c->record_energy && it < c->nt_max;
+      const bool rec = !s->quiet && c->record_energy && it < c->nt_max;
       if (rec && !s->ep_valid) {
         HIPCHK(s, tw::launch_stress_energy(p, s->stream));
         carry = tw::Pending();
@@ -995,7 +998,7 @@ int tw_step(tw_solver* s, int32_t nsteps) {
         cs.pend.it0 = it;
         cs.pend.scale = c->energy_form == 1 ? 1.0 : 0.5;
       }
-      if (it < c->nt_max && !s->rcv_v.empty()) {
+      if (!s->quiet && it < c->nt_max && !s->rcv_v.empty()) {
         cs.pend.rcv = p.rcv_v;
         cs.pend.nrcv = (int32_t)s->rcv_v.size();
         cs.pend.rcv_it = it;
@@ -1006,12 +1009,12 @@ int tw_step(tw_solver* s, int32_t nsteps) {
       swap_buffers(p);
       s->it++;
     }
-    if (have_carry) HIPCHK(s, tw::launch_finalize(p, carry, s->stream));
+    if (have_carry && (carry.nparts > 0 || carry.nrcv > 0)) HIPCHK(s, tw::launch_finalize(p, carry, s->stream));
     return TW_OK;
   }
   for (int k = 0; k < nsteps; k++) {
     const int it = s->it;
-    const bool rec = c->record_energy && it < c->nt_max;
+    const bool rec = !s->quiet && c->record_energy && it < c->nt_max;
     if (rec && !s->ep_valid) {
       HIPCHK(s, tw::launch_stress_energy(p, s->stream));
       carry = tw::Pending();
@@ -1055,7 +1058,7 @@ int tw_step(tw_solver* s, int32_t nsteps) {
       cs.pend.it0 = it;
       cs.pend.scale = c->energy_form == 1 ? 1.0 : 0.5;
     }
-    if (it < c->nt_max && !s->rcv_v.empty()) {
+    if (!s->quiet && it < c->nt_max && !s->rcv_v.empty()) {
       cs.pend.rcv = p.rcv_v;
       cs.pend.nrcv = (int32_t)s->rcv_v.size();
       cs.pend.rcv_it = it;
@@ -1067,7 +1070,7 @@ int tw_step(tw_solver* s, int32_t nsteps) {
     have_carry = true;
     s->it++;
   }
-  if (have_carry) HIPCHK(s, tw::launch_finalize(p, carry, s->stream));
+  if (have_carry && (carry.nparts > 0 || carry.nrcv > 0)) HIPCHK(s, tw::launch_finalize(p, carry, s->stream));
   return TW_OK;
 }
 
@@ -1697,10 +1700,13 @@ int adjoint_sweep(tw_solver* s, const double* rcv_adjoint, double* const* out) {
     s->it = it0 + b;
     s->ep_valid = false;
     if ((e = state_copy(s, s->adj_seg, true))) return e;
-    for (int t = b; t < en; t++) {
-      if ((e = tw_step(s, 1))) return e;
-      if ((e = state_copy(s, (char*)s->adj_seg + sb * (size_t)(t - b + 1), true))) return e;
+    s->quiet = true;
+    for (int t = b; t < en && !e; t++) {
+      e = tw_step(s, 1);
+      if (!e) e = state_copy(s, (char*)s->adj_seg + sb * (size_t)(t - b + 1), true);
     }
+    s->quiet = false;
+    if (e) return e;
     for (int t = en - 1; t >= b; t--) {
       const tw::StateView cur = state_view(s, (char*)s->adj_seg + sb * (size_t)(t - b + 1));
       const tw::StateView prev = state_view(s, (char*)s->adj_seg + sb * (size_t)(t - b));
