@@ -3424,6 +3424,22 @@ __device__ __forceinline__ bool tap_interior(int index, int len, int periodic) {
   return !periodic && index >= 5 && index <= len - 6;
 }
 
+// acc plus the transposed taps of partner index `index` (list L, a partner
+// array of len points along the axis), coefficient * f(output index) in the
+// list's order; interior indices without reading the list.  FIRST is -2 for
+// the M-output transposes (dt), -1 for the N-output ones (gt).
+template <int FIRST, class CT, class Fn>
+__device__ __forceinline__ CT tap_acc(CT acc, const TapT<CT>& L, int index, int len, int periodic, const CT st[4],
+                                      Fn f) {
+  if (tap_interior(index, len, periodic)) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc = acc + st[3 - k] * f(index + FIRST + k);
+  } else {
+    for (int e = 0; e < L.n[index]; e++) acc = acc + L.cf[index * TMAX + e] * f(L.idx[index * TMAX + e]);
+  }
+  return acc;
+}
+
 // K1
 template <class ST, class KT, class CT>
 __global__ void __launch_bounds__(256) k_adj1(Args<ST, KT, CT> U, AdjArgs<CT> J) {
@@ -3650,35 +3666,21 @@ __global__ void __launch_bounds__(256) k_eadj1(Args<ST, KT, CT> U, AdjArgs<CT> J
     J.g[TW_C_MU][q] += (double)mn[q] * d;
   }
   if (ix < X.rows && iy < X.cols) {  // mu_vx += Dx_M^T (lam (mu_sxx + mu_syy) + 2mu mu_sxx) + Dy_N^T (mu mu_sxy)
-    CT acc = (CT)0;
-    const TapT<CT>& L = J.dt[0];
-    for (int e = 0; e < L.n[ix]; e++) {
-      const int i = L.idx[ix * TMAX + e];
+    CT acc = tap_acc<-2>((CT)0, J.dt[0], ix, X.rows, U.perx, U.t.st, [&](int i) {
       const int64_t q = M.at(i, iy);
-      acc = acc + L.cf[ix * TMAX + e] *
-                      (kv(U, TW_C_LAMBDA, i, iy) * (m0[q] + m1[q]) + kv(U, TW_C_TWOMU, i, iy) * m0[q]);
-    }
-    const TapT<CT>& G = J.gt[1];
-    for (int e = 0; e < G.n[iy]; e++) {
-      const int j = G.idx[iy * TMAX + e];
-      acc = acc + G.cf[iy * TMAX + e] * (kv(U, TW_C_MU, ix, j) * mn[N.at(ix, j)]);
-    }
+      return kv(U, TW_C_LAMBDA, i, iy) * (m0[q] + m1[q]) + kv(U, TW_C_TWOMU, i, iy) * m0[q];
+    });
+    acc = tap_acc<-1>(acc, J.gt[1], iy, X.cols, U.pery, U.t.st,
+                      [&](int j) { return kv(U, TW_C_MU, ix, j) * mn[N.at(ix, j)]; });
     J.mu[TW_VX][X.at(ix, iy)] += acc;
   }
   if (ix < Y.rows && iy < Y.cols) {  // mu_vy += Dy_M^T (lam (mu_sxx + mu_syy) + 2mu mu_syy) + Dx_N^T (mu mu_sxy)
-    CT acc = (CT)0;
-    const TapT<CT>& L = J.dt[1];
-    for (int e = 0; e < L.n[iy]; e++) {
-      const int j = L.idx[iy * TMAX + e];
+    CT acc = tap_acc<-2>((CT)0, J.dt[1], iy, Y.cols, U.pery, U.t.st, [&](int j) {
       const int64_t q = M.at(ix, j);
-      acc = acc + L.cf[iy * TMAX + e] *
-                      (kv(U, TW_C_LAMBDA, ix, j) * (m0[q] + m1[q]) + kv(U, TW_C_TWOMU, ix, j) * m1[q]);
-    }
-    const TapT<CT>& G = J.gt[0];
-    for (int e = 0; e < G.n[ix]; e++) {
-      const int i = G.idx[ix * TMAX + e];
-      acc = acc + G.cf[ix * TMAX + e] * (kv(U, TW_C_MU, i, iy) * mn[N.at(i, iy)]);
-    }
+      return kv(U, TW_C_LAMBDA, ix, j) * (m0[q] + m1[q]) + kv(U, TW_C_TWOMU, ix, j) * m1[q];
+    });
+    acc = tap_acc<-1>(acc, J.gt[0], ix, Y.rows, U.perx, U.t.st,
+                      [&](int i) { return kv(U, TW_C_MU, i, iy) * mn[N.at(i, iy)]; });
     J.mu[TW_VY][Y.at(ix, iy)] += acc;
   }
 }
@@ -3715,12 +3717,8 @@ __global__ void __launch_bounds__(256) k_eadj2(Args<ST, KT, CT> U, AdjArgs<CT> J
   }
   if (ix < M.rows && iy < M.cols) {
     // mu_sxx += Dx_N^T (bx mu_vx) + the sends of the vx boundary rows (3-point projection)
-    CT acc = (CT)0;
-    const TapT<CT>& G = J.gt[0];
-    for (int e = 0; e < G.n[ix]; e++) {
-      const int j = G.idx[ix * TMAX + e];
-      acc = acc + G.cf[ix * TMAX + e] * (kv(U, TW_C_BINV_VX, j, iy) * ux[X.at(j, iy)]);
-    }
+    CT acc = tap_acc<-1>((CT)0, J.gt[0], ix, M.rows, U.perx, U.t.st,
+                         [&](int j) { return kv(U, TW_C_BINV_VX, j, iy) * ux[X.at(j, iy)]; });
     if (!U.perx) {
       if (U.bc[0] && ix < 3)
         acc = acc + (U.t.ainvN_L * U.t.projM_L[ix]) * (kv(U, TW_C_BINV_VX, 0, iy) * ux[X.at(0, iy)]);
@@ -3731,12 +3729,8 @@ __global__ void __launch_bounds__(256) k_eadj2(Args<ST, KT, CT> U, AdjArgs<CT> J
     }
     J.mu[TW_SXX][M.at(ix, iy)] += acc;
     // mu_syy += Dy_N^T (by mu_vy) + the sends of the vy boundary columns
-    acc = (CT)0;
-    const TapT<CT>& H = J.gt[1];
-    for (int e = 0; e < H.n[iy]; e++) {
-      const int j = H.idx[iy * TMAX + e];
-      acc = acc + H.cf[iy * TMAX + e] * (kv(U, TW_C_BINV_VY, ix, j) * uy[Y.at(ix, j)]);
-    }
+    acc = tap_acc<-1>((CT)0, J.gt[1], iy, M.cols, U.pery, U.t.st,
+                      [&](int j) { return kv(U, TW_C_BINV_VY, ix, j) * uy[Y.at(ix, j)]; });
     if (!U.pery) {
       if (U.bc[2] && iy < 3)
         acc = acc + (U.t.ainvN_L * U.t.projM_L[iy]) * (kv(U, TW_C_BINV_VY, ix, 0) * uy[Y.at(ix, 0)]);
@@ -3750,17 +3744,10 @@ __global__ void __launch_bounds__(256) k_eadj2(Args<ST, KT, CT> U, AdjArgs<CT> J
   if (ix < N.rows && iy < N.cols) {
     // mu_sxy += Dy_M^T (bx mu_vx) + Dx_M^T (by mu_vy) + the sends read by the
     // tangential velocities' 3 boundary points of each side
-    CT acc = (CT)0;
-    const TapT<CT>& L1 = J.dt[1];
-    for (int e = 0; e < L1.n[iy]; e++) {
-      const int j = L1.idx[iy * TMAX + e];
-      acc = acc + L1.cf[iy * TMAX + e] * (kv(U, TW_C_BINV_VX, ix, j) * ux[X.at(ix, j)]);
-    }
-    const TapT<CT>& L0 = J.dt[0];
-    for (int e = 0; e < L0.n[ix]; e++) {
-      const int i = L0.idx[ix * TMAX + e];
-      acc = acc + L0.cf[ix * TMAX + e] * (kv(U, TW_C_BINV_VY, i, iy) * uy[Y.at(i, iy)]);
-    }
+    CT acc = tap_acc<-2>((CT)0, J.dt[1], iy, N.cols, U.pery, U.t.st,
+                         [&](int j) { return kv(U, TW_C_BINV_VX, ix, j) * ux[X.at(ix, j)]; });
+    acc = tap_acc<-2>(acc, J.dt[0], ix, N.rows, U.perx, U.t.st,
+                      [&](int i) { return kv(U, TW_C_BINV_VY, i, iy) * uy[Y.at(i, iy)]; });
     if (!U.pery && ix < X.rows) {
       if (U.bc[2] && iy == 0)
         for (int w = 0; w < 3; w++) acc = acc + U.t.ainvM_L[w] * (kv(U, TW_C_BINV_VX, ix, w) * ux[X.at(ix, w)]);
