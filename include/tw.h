@@ -399,7 +399,8 @@ int tw_adjoint_forward(tw_solver* s, int32_t nsteps, int32_t interval);
  * tw_read_receivers divided by source_scaling), write dL/d(coefficient) as
  * dense [rows][cols] doubles on each coefficient's grid: kappa (MM), 1/rho on
  * NM, 1/rho on MN (null pointers skipped).  The fields are left as after the
- * forward run. */
+ * forward run, and so are its receiver and energy records (the steps the
+ * sweep recomputes record nothing). */
 int tw_adjoint_run(tw_solver* s, const double* rcv_adjoint, double* grad_kappa, double* grad_binv_vx,
                    double* grad_binv_vy);
 
