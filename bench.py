@@ -424,7 +424,7 @@ def main():
                     help="c4: coefficient storage (16-bit by default: the acoustic batch runs no faster with 8-bit)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (the slab decomposition "
+        print("bench.py runs on one GPU: the library has no multi-GPU path (the slab decomposition "
               "across GPUs is a design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch

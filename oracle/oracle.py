@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, where it is the checker.  The product library
+cpu_baseline leg of bench.py, where results are checked against it.  The product library
 (libtwave.so) never loads it.
 """
 import ctypes

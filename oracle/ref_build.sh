@@ -1,7 +1,7 @@
 #!/usr/bin/env bash
 # Builds the reference's acoustic CPU solvers from their own sources into
 # oracle/_ref/ and optionally runs them to produce golden receiver traces and
-# energy histories (test infrastructure: the checker and the CPU baseline).
+# energy histories (test infrastructure: golden vectors for the oracle and the CPU baseline).
 #
 # What builds from the reference's own few source files:
 #   * singlecore/  : header-only C++20, builds with clang++ (g++ 11 rejects the

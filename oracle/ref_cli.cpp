@@ -1,6 +1,6 @@
 // Command-line driver for the reference's single-core acoustic solver.
 //
-// TEST INFRASTRUCTURE ONLY (builds the checker's golden vectors; the product
+// TEST INFRASTRUCTURE ONLY (builds the oracle's golden vectors; the product
 // never uses it).  The reference's single-core program
 // (singlecore/src/main_cpp/cpu_acoustic.cpp) contains a command-line parser,
 // namespace_inputs::command_line_input_processing

@@ -1,7 +1,7 @@
 /*
  * tw_oracle.h -- CPU oracle for the staggered SBP-SAT wave-solver hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker: tests/, the smoke check and
+ * TEST INFRASTRUCTURE ONLY.  Results are checked against it: tests/, the smoke check and
  * bench.py's cpu_baseline leg load it; the product (libtwave.so) never links,
  * loads or calls it.
  *

@@ -1,7 +1,7 @@
 """Shared test setup: import paths and the `gpu` marker.
 
 GPU tests (marked `gpu`) call libtwave.so through the C-ABI on a HIP device;
-everything else runs on the CPU.  The CPU oracle (oracle/) is the checker.
+everything else runs on the CPU.  Results are checked against the CPU oracle (oracle/).
 """
 import os
 import sys

@@ -325,7 +325,7 @@ def test_adjoint_c5_full_scale(progress):
     on the first direction), so the differences are Richardson-extrapolated
     from h = 1e-5 and 2h (error O(h^4)).  For the same reason the directional
     derivatives are ~1e-7 of |g| |d|, below FP32 resolution of the gradient,
-    so FP32 is judged by its distance to the FP64 gradient."""
+    so FP32 is measured by its distance to the FP64 gradient."""
     import time
     import twave.workloads as W
     ndir = int(os.environ.get("TW_C5_DIRS", "32" if os.environ.get("TW_RUN_SLOW") else "4"))

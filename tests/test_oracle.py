@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/) before it is trusted as the parity checker.
+"""Pins the CPU oracle (oracle/) before parity is checked against it.
 
 Each check is chosen so that a plausible slip in the restatement (a dropped
 term, a wrong sign, shift or closure index, a transposed table, a wrong energy

@@ -40,7 +40,7 @@
 #define TW_PNAME(n, p) TW_PNAME_(n, p)
 #define TW_PUB(n) TW_PNAME(n, TW_PREC)
 
-// return F<ST, KT, CT>(args) for this build's storage type and the plan's
+// return F<ST, KT, CT>(args) for this unit's storage type (TW_PREC) and the plan's
 // coefficient storage (FP32, the 16-bit type, or 8-bit codes)
 #if TW_PREC == 0
 #define TW_DISPATCH(p, F, ...) return F<double, double, double>(__VA_ARGS__)
