@@ -203,6 +203,76 @@ class AdjointShots:
         self.s.close()
 
 
+class ElasticShots:
+    """The elastic counterpart of AdjointShots: n x n cells of the layered
+    model (`nlayers` layers, 5% correlated perturbation) with a seeded circular
+    anomaly of +10% in Vp and Vs as the true model; vertical-force Ricker shots
+    on vx (data (N,M)) `src_depth` cells below the free surface (top 'F',
+    characteristic elsewhere), receivers on vx along the surface, misfit
+    J = sum over shots and samples of (trace - data)^2 with data from the true
+    model.  The coefficients are (lambda, 2 mu, mu, 1/rho on NM, 1/rho on MN);
+    m0 is the layered model without the anomaly."""
+
+    def __init__(self, n=96, nt=600, nshots=2, nrcv=32, dx_km=0.005, f0=15.0, precision=FP64, seed=7,
+                 nlayers=6, src_depth=4, log=None):
+        from . import Solver, C_LAMBDA, C_MU, C_TWOMU
+        self.n, self.nt, self.log = n, nt, log
+        cfg = elastic_config(m=n, precision=precision, nt=nt, dx_km=dx_km)
+        self.s = Solver(cfg)
+        den = int(round(1.0 / dx_km))
+        rho, vp, vs = _layered(n, dx_km, seed, nlayers, (1.5, 4.0), 0.05, 0.05)
+        rng = np.random.default_rng(seed + 1)
+        cx, cy, rad = rng.uniform(0.15, 0.25) * n, rng.uniform(0.35, 0.65) * n, rng.uniform(0.06, 0.1) * n
+        X, Y = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+        bump = 1.0 + 0.1 * (((X - cx) ** 2 + (Y - cy) ** 2) < rad ** 2)
+        self.rho, self.vp0, self.vs0 = rho, vp, vs
+        self.vp_true, self.vs_true = vp * bump, vs * bump
+        self.anomaly = (cx, cy, rad)
+        self.src_depth = int(src_depth)
+        ry = np.linspace(4, n - 5, nrcv).round().astype(np.int32)
+        self.s.set_receivers(np.full(nrcv, VX, np.int32), np.zeros(nrcv, np.int32), ry)
+        self.shots = [int(v) for v in np.linspace(n // 4, n - n // 4, nshots)]
+        self.f0, self.t0 = f0, 1.25 / f0
+        self.state0 = self.s.checkpoint()
+        self.coefs = (C_LAMBDA, C_TWOMU, C_MU, C_BINV_VX, C_BINV_VY)
+        self.s.set_model(rho, self.vp_true, self.vs_true, 1, den)
+        truth = [self.s.get_coef(k) for k in self.coefs]
+        self.data = [self.run(truth, i)[0] for i in range(nshots)]
+        self.s.set_model(rho, vp, vs, 1, den)
+        self.m0 = [self.s.get_coef(k) for k in self.coefs]
+
+    def run(self, coefs, shot, residual_of=None):
+        """Traces of one shot for the coefficients; with residual_of (the
+        shot's data), also the adjoint gradient of sum (trace - data)^2, one
+        array per coefficient in `coefs` order."""
+        s = self.s
+        s.restore(self.state0)
+        for k, v in zip(self.coefs, coefs):
+            s.set_coef(k, v)
+        s.set_source_ricker(GRID_NM, self.src_depth, self.shots[shot], self.f0, self.t0)
+        if residual_of is None:
+            s.step(self.nt)
+            return s.receivers()[:, -self.nt:], None
+        s.adjoint_forward(self.nt)
+        tr = s.receivers()[:, -self.nt:]
+        g = s.adjoint_run_coefs(2.0 * (tr - residual_of))
+        return tr, [g[k] for k in self.coefs]
+
+    def misfit(self, coefs, grad=False):
+        """J and (with grad) its gradient, summed over the shots."""
+        total, g = 0.0, [np.zeros_like(m) for m in self.m0]
+        for i in range(len(self.shots)):
+            tr, gi = self.run(coefs, i, self.data[i] if grad else None)
+            total += float(np.sum((tr - self.data[i]) ** 2))
+            if grad:
+                for a, b in zip(g, gi):
+                    a += b
+        return total, g
+
+    def close(self):
+        self.s.close()
+
+
 def smooth_direction(rng, shape, corr):
     """Unit-peak random model perturbation, Gaussian-correlated over `corr`
     cells."""
