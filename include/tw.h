@@ -144,6 +144,16 @@ int tw_interpolate_model(const tw_config* cfg, int32_t grid, const double* model
                          int32_t nxm, int32_t nym, int64_t num_dxm, int64_t den_dxm,
                          double* out);
 
+/* Transpose of tw_interpolate_model: model_out[(nxm+1)][(nym+1)] = the sum
+ * over the grid's points of each bilinear weight times values[point], so that
+ * gradients on a staggered grid become gradients on the model nodes.  No
+ * counterpart in the reference (it has no gradient); it transposes
+ * Class_Grid::interpolate_forward_parameter (cuda/src/cpp/grid_parameter.cpp:6-60)
+ * for the model parameters of Class_Inverse_Specs (cuda/src/cpp/inverse.hpp:10). */
+int tw_interpolate_model_adjoint(const tw_config* cfg, int32_t grid, const double* values,
+                                 int32_t nxm, int32_t nym, int64_t num_dxm, int64_t den_dxm,
+                                 double* model_out);
+
 /* Ricker wavelet increment A*dt/dx^2/source_scaling for a step; replaces the
  * source evaluation in forward_simulation_periodic_y
  * (cuda/src/cpp/forward_simulation.cpp:62-70,104-112). half = 0 for velocity

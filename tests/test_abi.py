@@ -119,3 +119,26 @@ def test_ricker_matches_oracle():
     for it in (0, 1, 10, 2499, 2500, 10000):
         for half in (0, 1):
             assert T.ricker_increment(c, 5.0, 0.25, it, half) == O.ricker_increment(oc, 5.0, 0.25, it, half)
+
+
+@pytest.mark.parametrize("spacing", [(1, 100, 41, 31), (1, 50, 21, 16), (3, 200, 28, 21)])
+def test_interpolation_adjoint_is_the_transpose(spacing):
+    """<interp(x), y> = <x, interp^T(y)> on every grid, model spacings equal
+    to, twice and 1.5 times the grid's (nodes beyond the model included), and
+    interp^T of one grid point is that point's four weights."""
+    num, den, nx, ny = spacing
+    c = T.config(mx=40, my=30, num_dx=1, den_dx=100)
+    rng = np.random.default_rng(7)
+    for g in range(4):
+        x = rng.standard_normal((nx, ny))
+        y = rng.standard_normal(T.interpolate_model(c, g, x, num, den).shape)
+        lhs = float(np.sum(T.interpolate_model(c, g, x, num, den) * y))
+        rhs = float(np.sum(x * T.interpolate_model_adjoint(c, g, y, nx - 1, ny - 1, num, den)))
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1.0), (g, lhs, rhs)
+        e = np.zeros_like(y)
+        e[3, 5] = 1.0
+        col = T.interpolate_model_adjoint(c, g, e, nx - 1, ny - 1, num, den)
+        assert abs(col.sum() - 1.0) < 1e-15 and np.count_nonzero(col) <= 4
+        np.testing.assert_allclose(np.sum(col * x), T.interpolate_model(c, g, x, num, den)[3, 5], rtol=1e-14)
+    with pytest.raises(T.TwError):  # a model too small to cover the grid
+        T.interpolate_model_adjoint(c, 0, np.zeros((41, 31)), 5, 5, 1, 100)

@@ -62,3 +62,45 @@ def test_elastic_steepest_descent_decreases_misfit(problem):
     # only the moduli moved
     for i in (3, 4):
         np.testing.assert_array_equal(opt.coefs[i], problem.m0[i])
+
+
+def test_model_gradient_matches_fd(problem):
+    """Gradient with respect to the model of set_model (rho, Vp, Vs on the
+    model nodes, the reference's inversion parameters, inverse.hpp:10) by the
+    chain rule through the transposed interpolation (twave.model_gradient),
+    against Richardson-extrapolated central differences of the misfit of the
+    models themselves."""
+    print("case elastic model gradient", flush=True)
+    P = problem
+    den = int(round(1.0 / P.s.cfg.dx))
+    model0 = (P.rho, P.vp0, P.vs0)
+
+    def coefs_of(model):
+        P.s.set_model(*model, 1, den)
+        return [P.s.get_coef(k) for k in P.coefs]
+
+    _, g = P.misfit(coefs_of(model0), grad=True)
+    gm = T.model_gradient(P.s.cfg, dict(zip(P.coefs, g)), *model0, 1, den)
+    rng = np.random.default_rng(9)
+    d = [m * W.smooth_direction(rng, m.shape, 8) for m in model0]
+
+    def cd(h):
+        jp = P.misfit(coefs_of([m + h * e for m, e in zip(model0, d)]))[0]
+        jm = P.misfit(coefs_of([m - h * e for m, e in zip(model0, d)]))[0]
+        return (jp - jm) / (2 * h)
+
+    h = 1e-5
+    fd = (4 * cd(h / 2) - cd(h)) / 3
+    ad = sum(float(np.sum(a * b)) for a, b in zip(gm, d))
+    assert abs(fd - ad) <= 1e-7 * abs(ad), (fd, ad)
+    # each parameter on its own
+    for i in range(3):
+        di = [np.zeros_like(m) for m in model0]
+        di[i] = d[i]
+        jp = P.misfit(coefs_of([m + h * e for m, e in zip(model0, di)]))[0]
+        jm = P.misfit(coefs_of([m - h * e for m, e in zip(model0, di)]))[0]
+        jp2 = P.misfit(coefs_of([m + 0.5 * h * e for m, e in zip(model0, di)]))[0]
+        jm2 = P.misfit(coefs_of([m - 0.5 * h * e for m, e in zip(model0, di)]))[0]
+        fdi = (4 * (jp2 - jm2) / h - (jp - jm) / (2 * h)) / 3
+        adi = float(np.sum(gm[i] * d[i]))
+        assert abs(fdi - adi) <= 1e-6 * abs(adi), (i, fdi, adi)

@@ -247,29 +247,78 @@ int locate_1d(int64_t num, int64_t den, int64_t num_dx, int64_t den_dx, int node
 }
 
 // cuda/src/cpp/grid_parameter.cpp:6-60 (bilinear, fixed term order)
+// Model node (ixm, iym) and bilinear weights (xi, eta) of point (ix, iy) of
+// grid g; false when the model does not cover the point.
+struct InterpAt {
+  int64_t ixm, iym;
+  double xi, eta;
+};
+bool interp_at(const tw_config* c, int g, int ix, int iy, int64_t num_dxm, int64_t den_dxm, int64_t Nx, int64_t Ny,
+               InterpAt* w) {
+  const int64_t nd = c->num_dx, dd = c->den_dx;
+  int64_t nxl, dxl, nyl, dyl;
+  if (grid_tx(g)) { nxl = ix * nd; dxl = dd; } else { nxl = (2 * (int64_t)ix + 1) * nd; dxl = 2 * dd; }
+  if (grid_ty(g)) { nyl = iy * nd; dyl = dd; } else { nyl = (2 * (int64_t)iy + 1) * nd; dyl = 2 * dd; }
+  w->ixm = (nxl * den_dxm) / (dxl * num_dxm);
+  w->iym = (nyl * den_dxm) / (dyl * num_dxm);
+  w->xi = (double)(nxl * den_dxm) / (double)(dxl * num_dxm) - w->ixm;
+  w->eta = (double)(nyl * den_dxm) / (double)(dyl * num_dxm) - w->iym;
+  return !(w->ixm < 0 || w->ixm + 1 > Nx || w->iym < 0 || w->iym + 1 > Ny);
+}
+
+int interp_check(const tw_config* c, int32_t nxm, int32_t nym, int64_t num_dxm, int64_t den_dxm) {
+  if (c->num_dx <= 0 || c->den_dx <= 0)
+    return fail(nullptr, TW_EINVAL, "tw_config num_dx/den_dx must be set for model interpolation");
+  if (num_dxm <= 0 || den_dxm <= 0 || nxm < 1 || nym < 1) return fail(nullptr, TW_EINVAL, "model spacing/size");
+  return TW_OK;
+}
+
 int interpolate(const tw_config* c, int g, const double* model, int32_t nxm, int32_t nym, int64_t num_dxm,
                 int64_t den_dxm, double* out) {
   int rows, cols;
   shape_of(c, g, &rows, &cols);
   const int64_t Nx = (int64_t)nxm + 1, Ny = (int64_t)nym + 1;
-  const int64_t nd = c->num_dx, dd = c->den_dx;
-  if (nd <= 0 || dd <= 0) return fail(nullptr, TW_EINVAL, "tw_config num_dx/den_dx must be set for model interpolation");
-  if (num_dxm <= 0 || den_dxm <= 0 || nxm < 1 || nym < 1) return fail(nullptr, TW_EINVAL, "model spacing/size");
+  int e = interp_check(c, nxm, nym, num_dxm, den_dxm);
+  if (e) return e;
   auto mv = [&](int64_t a, int64_t b) { return (a < Nx && b < Ny) ? model[a * Ny + b] : 0.0; };
   for (int ix = 0; ix < rows; ix++)
     for (int iy = 0; iy < cols; iy++) {
-      int64_t nxl, dxl, nyl, dyl;
-      if (grid_tx(g)) { nxl = ix * nd; dxl = dd; } else { nxl = (2 * (int64_t)ix + 1) * nd; dxl = 2 * dd; }
-      if (grid_ty(g)) { nyl = iy * nd; dyl = dd; } else { nyl = (2 * (int64_t)iy + 1) * nd; dyl = 2 * dd; }
-      const int64_t ixm = (nxl * den_dxm) / (dxl * num_dxm);
-      const int64_t iym = (nyl * den_dxm) / (dyl * num_dxm);
-      const double xi = (double)(nxl * den_dxm) / (double)(dxl * num_dxm) - ixm;
-      const double eta = (double)(nyl * den_dxm) / (double)(dyl * num_dxm) - iym;
-      if (ixm < 0 || ixm + 1 > Nx || iym < 0 || iym + 1 > Ny)
+      InterpAt w;
+      if (!interp_at(c, g, ix, iy, num_dxm, den_dxm, Nx, Ny, &w))
         return fail(nullptr, TW_ERANGE, "model does not cover the simulation grid");
+      const int64_t ixm = w.ixm, iym = w.iym;
+      const double xi = w.xi, eta = w.eta;
       const double v = (1 - xi) * (1 - eta) * mv(ixm, iym) + (1 - xi) * (eta)*mv(ixm, iym + 1) +
                        (xi) * (1 - eta) * mv(ixm + 1, iym) + (xi) * (eta)*mv(ixm + 1, iym + 1);
       out[(size_t)ix * cols + iy] = v;
+    }
+  return TW_OK;
+}
+
+// transpose of interpolate: out (model nodes) = sum over the grid's points of
+// each weight times the point's value (nodes beyond the model read as 0 by
+// interpolate receive nothing)
+int interpolate_adjoint(const tw_config* c, int g, const double* vals, int32_t nxm, int32_t nym, int64_t num_dxm,
+                        int64_t den_dxm, double* out) {
+  int rows, cols;
+  shape_of(c, g, &rows, &cols);
+  const int64_t Nx = (int64_t)nxm + 1, Ny = (int64_t)nym + 1;
+  int e = interp_check(c, nxm, nym, num_dxm, den_dxm);
+  if (e) return e;
+  std::fill(out, out + Nx * Ny, 0.0);
+  auto add = [&](int64_t a, int64_t b, double v) {
+    if (a < Nx && b < Ny) out[a * Ny + b] += v;
+  };
+  for (int ix = 0; ix < rows; ix++)
+    for (int iy = 0; iy < cols; iy++) {
+      InterpAt w;
+      if (!interp_at(c, g, ix, iy, num_dxm, den_dxm, Nx, Ny, &w))
+        return fail(nullptr, TW_ERANGE, "model does not cover the simulation grid");
+      const double y = vals[(size_t)ix * cols + iy];
+      add(w.ixm, w.iym, (1 - w.xi) * (1 - w.eta) * y);
+      add(w.ixm, w.iym + 1, (1 - w.xi) * (w.eta) * y);
+      add(w.ixm + 1, w.iym, (w.xi) * (1 - w.eta) * y);
+      add(w.ixm + 1, w.iym + 1, (w.xi) * (w.eta) * y);
     }
   return TW_OK;
 }
@@ -583,6 +632,12 @@ int tw_interpolate_model(const tw_config* cfg, int32_t grid, const double* model
                          int64_t num_dxm, int64_t den_dxm, double* out) {
   if (!cfg || !model || !out || grid < 0 || grid > 3) return fail(nullptr, TW_EINVAL, "bad argument");
   return interpolate(cfg, grid, model, nxm, nym, num_dxm, den_dxm, out);
+}
+
+int tw_interpolate_model_adjoint(const tw_config* cfg, int32_t grid, const double* values, int32_t nxm, int32_t nym,
+                                 int64_t num_dxm, int64_t den_dxm, double* model_out) {
+  if (!cfg || !values || !model_out || grid < 0 || grid > 3) return fail(nullptr, TW_EINVAL, "bad argument");
+  return interpolate_adjoint(cfg, grid, values, nxm, nym, num_dxm, den_dxm, model_out);
 }
 
 double tw_ricker_increment(const tw_config* cfg, double f0, double t0, int32_t it, int32_t half) {

@@ -103,6 +103,8 @@ _SIGS = [
                                  ctypes.c_int64, _ip, _ip]),
     ("tw_interpolate_model", ctypes.c_int, [_cp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int64, ctypes.c_int64, _dp]),
+    ("tw_interpolate_model_adjoint", ctypes.c_int, [_cp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32,
+                                                    ctypes.c_int64, ctypes.c_int64, _dp]),
     ("tw_ricker_increment", ctypes.c_double, [_cp, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
                                               ctypes.c_int32]),
     ("tw_global_error", ctypes.c_char_p, []),
@@ -250,6 +252,65 @@ def interpolate_model(cfg, grid, model, num_dxm, den_dxm):
     if e:
         raise TwError(e, lib().tw_global_error().decode())
     return out
+
+
+def interpolate_model_adjoint(cfg, grid, values, nxm, nym, num_dxm, den_dxm):
+    """Transpose of interpolate_model: values on one staggered grid -> the
+    (nxm+1) x (nym+1) model nodes (tw_interpolate_model_adjoint)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.zeros((nxm + 1, nym + 1))
+    e = lib().tw_interpolate_model_adjoint(ctypes.byref(cfg), grid, _d(v), nxm, nym, num_dxm, den_dxm, _d(out))
+    if e:
+        raise TwError(e, lib().tw_global_error().decode())
+    return out
+
+
+# grid of each material coefficient
+COEF_GRID = {C_KAPPA: GRID_MM, C_LAMBDA: GRID_MM, C_TWOMU: GRID_MM, C_MU: GRID_NN, C_BINV_VX: GRID_NM,
+             C_BINV_VY: GRID_MN}
+
+
+def model_gradient(cfg, grads, rho, vp, vs, num_dxm, den_dxm):
+    """Gradients with respect to the coefficients ({coefficient id: gradient
+    on its grid}, as adjoint_run_coefs returns them) -> gradients with respect
+    to the model of set_model on its nodes: (d/d rho, d/d vp, d/d vs), the
+    last None for acoustic.  set_model forms the moduli on the nodes and
+    interpolates them (kappa = rho vp^2; lambda = rho vp^2 - 2 rho vs^2,
+    2 mu = 2 rho vs^2, mu = rho vs^2) and takes 1/rho as one over the
+    interpolated density, so the chain rule runs through the transposed
+    interpolation (the coefficient rounding of the storage type is taken as
+    the identity)."""
+    rho, vp = np.asarray(rho, np.float64), np.asarray(vp, np.float64)
+    vs = None if vs is None else np.asarray(vs, np.float64)
+    nxm, nym = rho.shape[0] - 1, rho.shape[1] - 1
+
+    def back(k, v):
+        return interpolate_model_adjoint(cfg, COEF_GRID[k], v, nxm, nym, num_dxm, den_dxm)
+
+    g_rho, g_vp = np.zeros_like(rho), np.zeros_like(rho)
+    g_vs = None if vs is None else np.zeros_like(rho)
+    if C_KAPPA in grads:
+        G = back(C_KAPPA, grads[C_KAPPA])
+        g_rho += G * vp ** 2
+        g_vp += G * 2 * rho * vp
+    if C_LAMBDA in grads:
+        G = back(C_LAMBDA, grads[C_LAMBDA])
+        g_rho += G * (vp ** 2 - 2 * vs ** 2)
+        g_vp += G * 2 * rho * vp
+        g_vs += G * (-4 * rho * vs)
+    if C_TWOMU in grads:
+        G = back(C_TWOMU, grads[C_TWOMU])
+        g_rho += G * 2 * vs ** 2
+        g_vs += G * 4 * rho * vs
+    if C_MU in grads:
+        G = back(C_MU, grads[C_MU])
+        g_rho += G * vs ** 2
+        g_vs += G * 2 * rho * vs
+    for k in (C_BINV_VX, C_BINV_VY):
+        if k in grads:
+            ri = interpolate_model(cfg, COEF_GRID[k], rho, num_dxm, den_dxm)
+            g_rho += back(k, -np.asarray(grads[k]) / ri ** 2)
+    return g_rho, g_vp, g_vs
 
 
 def ricker_increment(cfg, f0, t0, it, half):
