@@ -104,3 +104,31 @@ def test_model_gradient_matches_fd(problem):
         fdi = (4 * (jp2 - jm2) / h - (jp - jm) / (2 * h)) / 3
         adi = float(np.sum(gm[i] * d[i]))
         assert abs(fdi - adi) <= 1e-6 * abs(adi), (i, fdi, adi)
+
+
+def test_model_traces_autograd(problem):
+    """twave.autograd.ModelTraces: the traces of one shot as a function of the
+    model tensors (rho, Vp, Vs); backward equals model_gradient of the C-ABI
+    gradient of the same misfit, and a second forward before the first
+    backward does not change it."""
+    print("case elastic model autograd", flush=True)
+    import torch
+    import twave.autograd as TA
+    P = problem
+    den = int(round(1.0 / P.s.cfg.dx))
+    model0 = (P.rho, P.vp0, P.vs0)
+    P.s.set_model(*model0, 1, den)
+    coefs = [P.s.get_coef(k) for k in P.coefs]
+    _, g = P.run(coefs, 0, P.data[0])
+    want = T.model_gradient(P.s.cfg, dict(zip(P.coefs, g)), *model0, 1, den)
+    P.s.restore(P.state0)
+    P.s.set_source_ricker(T.GRID_NM, P.src_depth, P.shots[0], P.f0, P.t0)
+    op = TA.ModelTraces(P.s, P.nt, 1, den)
+    m = [torch.tensor(a, requires_grad=True) for a in model0]
+    data = torch.from_numpy(P.data[0])
+    loss = ((op(*m) - data) ** 2).sum()
+    other = ((op(*[t.detach() * 1.01 for t in m]) - data) ** 2).sum()  # a later forward
+    loss.backward()
+    assert float(loss.detach()) > 0 and float(other) != float(loss.detach())
+    for t, w in zip(m, want):
+        np.testing.assert_allclose(t.grad.numpy(), w, rtol=1e-12, atol=1e-12 * np.abs(w).max())

@@ -1,7 +1,9 @@
 """torch.autograd.Function over the solver: receiver traces as a
 differentiable function of the material coefficients.  Acoustic: kappa on MM,
 1/rho on NM and MN (or kappa and rho itself).  Elastic: lambda and 2 mu on
-MM, mu on NN, 1/rho on NM and MN (or lambda, mu and rho).  Forward =
+MM, mu on NN, 1/rho on NM and MN (or lambda, mu and rho).  Either physics:
+the model of set_model, rho, Vp and Vs on the model nodes (ModelTraces).
+Forward =
 checkpointed forward run (tw_adjoint_forward), backward = the hand-written
 discrete adjoint (tw_adjoint_run_coefs) with the incoming trace gradient as
 the receiver sources.
@@ -20,7 +22,7 @@ import itertools
 import numpy as np
 import torch
 
-from . import C_BINV_VX, C_BINV_VY, C_KAPPA, C_LAMBDA, C_MU, C_TWOMU
+from . import C_BINV_VX, C_BINV_VY, C_KAPPA, C_LAMBDA, C_MU, C_TWOMU, ELASTIC, model_gradient
 
 _ids = itertools.count(1)
 
@@ -105,3 +107,49 @@ class ElasticTracesLameRho(ElasticTraces):
 
     def __call__(self, lam, mu_mm, mu_nn, rho_vx, rho_vy):
         return _Traces.apply(self, lam, 2.0 * mu_mm, mu_nn, 1.0 / rho_vx, 1.0 / rho_vy)
+
+
+class _ModelTraces(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, op, rho, vp, vs):
+        model = [t.detach().to("cpu", torch.float64).numpy().copy() for t in (rho, vp, vs)]
+        op.solver.set_model(*model, op.num_dxm, op.den_dxm)
+        coefs = [op.solver.get_coef(k) for k in op.coefs]
+        _run_forward(op, coefs)
+        fid = next(_ids)
+        op.last_forward = fid
+        tr = op.solver.receivers()[:, -op.nsteps:]
+        ctx.op, ctx.fid, ctx.coefs, ctx.model, ctx.like = op, fid, coefs, model, rho
+        return torch.from_numpy(np.ascontiguousarray(tr)).to(device=rho.device, dtype=rho.dtype)
+
+    @staticmethod
+    def backward(ctx, grad):
+        op = ctx.op
+        if op.last_forward != ctx.fid:
+            _run_forward(op, ctx.coefs)
+            op.last_forward = ctx.fid
+        g = op.solver.adjoint_run_coefs(grad.detach().to("cpu", torch.float64).numpy())
+        rho, vp, vs = ctx.model
+        out = model_gradient(op.solver.cfg, g, rho, vp, vs if op.solver.cfg.physics == ELASTIC else None,
+                             op.num_dxm, op.den_dxm)
+        gs = out[2] if out[2] is not None else np.zeros_like(rho)
+        like = ctx.like
+        return (None,) + tuple(torch.from_numpy(np.ascontiguousarray(a)).to(device=like.device, dtype=like.dtype)
+                               for a in (out[0], out[1], gs))
+
+
+class ModelTraces(_Op):
+    """traces = ModelTraces(solver, nsteps, num_dxm, den_dxm)(rho, vp, vs): the
+    model of set_model on its (nxm+1) x (nym+1) nodes with spacing
+    num_dxm/den_dxm (the reference's inversion parameters rho, Vp, Vs,
+    cuda/src/cpp/inverse.hpp:10); each forward sets the model, backward maps
+    the coefficient gradients to the nodes (twave.model_gradient).  Acoustic
+    solvers ignore vs (its gradient is zero)."""
+
+    def __init__(self, solver, nsteps, num_dxm, den_dxm, interval=0):
+        super().__init__(solver, nsteps, interval)
+        self.num_dxm, self.den_dxm = int(num_dxm), int(den_dxm)
+        self.coefs = ELASTIC_COEFS if solver.cfg.physics == ELASTIC else ACOUSTIC_COEFS
+
+    def __call__(self, rho, vp, vs):
+        return _ModelTraces.apply(self, rho, vp, vs)
