@@ -214,9 +214,9 @@ class ElasticShots:
     m0 is the layered model without the anomaly."""
 
     def __init__(self, n=96, nt=600, nshots=2, nrcv=32, dx_km=0.005, f0=15.0, precision=FP64, seed=7,
-                 nlayers=6, src_depth=4, log=None):
+                 nlayers=6, src_depth=4):
         from . import Solver, C_LAMBDA, C_MU, C_TWOMU
-        self.n, self.nt, self.log = n, nt, log
+        self.n, self.nt = n, nt
         cfg = elastic_config(m=n, precision=precision, nt=nt, dx_km=dx_km)
         self.s = Solver(cfg)
         den = int(round(1.0 / dx_km))
