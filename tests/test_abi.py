@@ -142,3 +142,45 @@ def test_interpolation_adjoint_is_the_transpose(spacing):
         np.testing.assert_allclose(np.sum(col * x), T.interpolate_model(c, g, x, num, den)[3, 5], rtol=1e-14)
     with pytest.raises(T.TwError):  # a model too small to cover the grid
         T.interpolate_model_adjoint(c, 0, np.zeros((41, 31)), 5, 5, 1, 100)
+
+
+@pytest.mark.parametrize("physics", [T.ACOUSTIC, T.ELASTIC])
+def test_model_gradient_chain_rule(physics):
+    """twave.model_gradient against central differences of a linear functional
+    of the coefficients formed as tw_set_model forms them (moduli on the model
+    nodes, then interpolated; 1/rho after the interpolation), host-only."""
+    c = T.config(physics=physics, mx=24, my=20, num_dx=1, den_dx=100)
+    rng = np.random.default_rng(5)
+    shp = (25, 21)
+    rho, vp = rng.uniform(1.0, 2.0, shp), rng.uniform(2.0, 4.0, shp)
+    vs = vp / 1.73 if physics == T.ELASTIC else None
+    ids = (T.C_LAMBDA, T.C_TWOMU, T.C_MU, T.C_BINV_VX, T.C_BINV_VY) if physics == T.ELASTIC else \
+        (T.C_KAPPA, T.C_BINV_VX, T.C_BINV_VY)
+
+    def coefs(r, p, s):
+        node = {T.C_KAPPA: r * p ** 2, T.C_MU: r * s ** 2 if s is not None else None}
+        if s is not None:
+            node[T.C_TWOMU] = 2 * r * s ** 2
+            node[T.C_LAMBDA] = r * p ** 2 - 2 * r * s ** 2
+        out = {}
+        for k in ids:
+            g = T.COEF_GRID[k]
+            out[k] = 1.0 / T.interpolate_model(c, g, r, 1, 100) if k in (T.C_BINV_VX, T.C_BINV_VY) else \
+                T.interpolate_model(c, g, node[k], 1, 100)
+        return out
+
+    c0 = coefs(rho, vp, vs)
+    W = {k: rng.standard_normal(v.shape) for k, v in c0.items()}
+    gr, gp, gs = T.model_gradient(c, W, rho, vp, vs, 1, 100)
+    assert (gs is None) == (physics == T.ACOUSTIC)
+    d = [rng.standard_normal(shp) for _ in range(3)]
+
+    def F(h):
+        r, p = rho + h * d[0], vp + h * d[1]
+        s = None if vs is None else vs + h * d[2]
+        return sum(float(np.sum(W[k] * v)) for k, v in coefs(r, p, s).items())
+
+    h = 1e-6
+    fd = (F(h) - F(-h)) / (2 * h)
+    ad = float(np.sum(gr * d[0]) + np.sum(gp * d[1]) + (0.0 if gs is None else np.sum(gs * d[2])))
+    assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
