@@ -388,6 +388,43 @@ def c4_batch(args):
     print(json.dumps(out))
 
 
+def c5_adjoint(args):
+    """The adjoint-gradient configuration: acoustic 2048^2, 2 km/s background
+    plus a seeded circular 10% anomaly, Ricker 12 Hz shots on p under the free
+    surface, 512 receivers, 3000 steps (twave.workloads.AdjointShots); the
+    FP64 data of --c5-shots shots, then the gradient of each shot's misfit
+    (checkpointed forward, backward sweep) timed in the storage of
+    --c5-precision from the background model.  value = gradients per second;
+    the per-step split comes from rocprofv3 of this command."""
+    import torch
+    import twave as T
+    import twave.workloads as W
+    torch.cuda.set_device(0)
+    n, nt, k = 2048, 3000, args.c5_shots
+    prec = T.PRECISION[args.c5_precision]
+    ref = W.AdjointShots(n=n, nt=nt, nshots=16, nrcv=512, precision=T.FP64)
+    P = ref if prec == T.FP64 else W.AdjointShots(n=n, nt=nt, nshots=16, nrcv=512, precision=prec, data=ref.data)
+    t = []
+    for i in range(k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.run(P.m0, i, P.data[i])
+        torch.cuda.synchronize()
+        t.append(time.perf_counter() - t0)
+    if P is not ref:
+        P.close()
+    ref.close()
+    sec = float(np.mean(t))
+    out = {"metric": "adjoint gradients per second (acoustic 2048^2, 3000 steps, 512 receivers)",
+           "value": 1.0 / sec, "unit": "shot gradients/s", "n_gpus": 1, "steps": nt, "warmup": 0,
+           "ms_per_step": sec / nt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": args.c5_precision, "data": "synthetic",
+           "config": {"workload": "acoustic 2048x2048, dx 10 m, 2 km/s + 10% circular anomaly, FCCC SAT, "
+                                  "Ricker 12 Hz shots on p at row 4, 512 receivers, 3000 steps, sqrt(N) checkpoints",
+                      "shots_timed": k, "seconds_per_shot": [round(v, 4) for v in t]}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -416,8 +453,11 @@ def main():
                     help="copy of --copy-calib: torch's, or the library's non-temporal streaming copy")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
-    ap.add_argument("--workload", default="elastic", choices=["elastic", "c4"],
-                    help="elastic: the mixed-precision elastic case (default); c4: batched acoustic shots")
+    ap.add_argument("--workload", default="elastic", choices=["elastic", "c4", "c5"],
+                    help="elastic: the mixed-precision elastic case (default); c4: batched acoustic shots; "
+                    "c5: adjoint gradients of the 2048^2 acoustic shots")
+    ap.add_argument("--c5-shots", type=int, default=2, help="c5: shot gradients timed")
+    ap.add_argument("--c5-precision", default="fp32", choices=["fp32", "fp64"], help="c5: storage")
     ap.add_argument("--shots", type=int, default=64, help="c4: number of problems")
     ap.add_argument("--c4-steps", type=int, default=1500, help="c4: time steps")
     ap.add_argument("--c4-coef16", type=int, default=1,
@@ -433,6 +473,9 @@ def main():
         return
     if args.workload == "c4":
         c4_batch(args)
+        return
+    if args.workload == "c5":
+        c5_adjoint(args)
         return
     import twave as T
     import twave.workloads as W
